@@ -1,0 +1,267 @@
+"""Write the golden fixtures in this folder from the reference ssqueezepy.
+
+Run by hand, in a container where the reference source tree is mounted at
+/root/reference (it is never needed at test time):
+
+    python tests/golden/make_golden.py
+
+The reference's jitted functions are executed as plain Python: numba is not
+installed here, so a stand-in ``numba`` module whose ``jit``/``njit`` return
+the function unchanged and whose ``prange`` is ``range`` is registered first.
+This is the reference's own no-JIT mode (tests/z_all_test.py:8-20 patches
+numba the same way).  cupy is absent, so the reference stays on its NumPy CPU
+path.
+
+What is written (all small .npz files, inputs + expected outputs):
+  grids.npz     scales / ssq_freqs / nv / squeeze bin parameters for the
+                configurations the benchmarks and tests use
+                (utils/cwt_utils.py:196-580, ssqueezing.py:228-310,
+                algos.py:44-123,356-374)
+  e2e_cwt.npz   full ssq_cwt / cwt outputs at N=100 (_ssq_cwt.py:12-310)
+  e2e_stft.npz  full ssq_stft / stft outputs at N=100 (_ssq_stft.py:13-136)
+  squeeze.npz   ssqueeze_fast / indexed_sum_onfly / phase_* on seeded random
+                inputs, following tests/fft_test.py:141-377
+  c1.npz        the N=10,000 Morlet(mu=6) chirp+tone+noise case in float64:
+                row norms and sampled columns of Tx, Wx, w
+  windows.npz   stft windows and their frequency-domain derivatives
+                (_stft.py:259-310)
+"""
+import os
+import sys
+import types
+import numpy as np
+
+REF = '/root/reference'
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def _import_reference():
+    nb = types.ModuleType('numba')
+
+    def jit(*args, **kw):
+        if args and callable(args[0]) and not kw:
+            return args[0]
+        return lambda fn: fn
+    nb.jit = jit
+    nb.njit = jit
+    nb.prange = range
+    sys.modules['numba'] = nb
+    sys.path.insert(0, REF)
+    import ssqueezepy
+    return ssqueezepy
+
+
+def chirp_tone_noise(N, seed):
+    """Linear chirp 0.02->0.2 cycles/sample + 0.5 * tone at 0.3 + 0.1 * noise,
+    float32.  Same recipe as ssqueezepy_amd.signals.chirp_tone_noise."""
+    n = np.arange(N, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    x = (np.cos(2 * np.pi * (0.02 * n + 0.09 * n**2 / N))
+         + 0.5 * np.cos(2 * np.pi * 0.3 * n)
+         + 0.1 * rng.standard_normal(N))
+    return x.astype(np.float32)
+
+
+def main():
+    sq = _import_reference()
+    from ssqueezepy.utils import process_scales
+    from ssqueezepy.ssqueezing import _compute_associated_frequencies
+    from ssqueezepy.algos import (_get_params_find_closest_log,
+                                  ssqueeze_fast, indexed_sum_onfly,
+                                  phase_cwt_cpu, phase_stft_cpu)
+    from ssqueezepy._stft import get_window
+    from ssqueezepy._ssq_stft import _make_Sfs
+
+    # ---------------------------------------------------------------- grids
+    grids = {}
+    cases = []
+    for dt in ('float32', 'float64'):
+        for N in (10000, 4096, 5000, 6000, 17000, 33000, 65536, 100, 300):
+            cases.append((f'morlet6_{dt}_lp_log_{N}', ('morlet', {'mu': 6., 'dtype': dt}),
+                          N, 'log-piecewise', 'log', 'peak'))
+        for N in (160000, 16384, 1048576, 10000, 512, 100):
+            cases.append((f'gmw_{dt}_lp_lp_{N}', ('gmw', {'dtype': dt}),
+                          N, 'log-piecewise', None, 'peak'))
+        cases.append((f'gmw_{dt}_log_log_2048', ('gmw', {'dtype': dt}),
+                      2048, 'log', None, 'peak'))
+        cases.append((f'gmw_{dt}_lin_lin_2048', ('gmw', {'dtype': dt}),
+                      2048, 'linear', None, 'peak'))
+        cases.append((f'gmw_{dt}_log_log_maximal_2048', ('gmw', {'dtype': dt}),
+                      2048, 'log', 'log', 'maximal'))
+        cases.append((f'morlet134_{dt}_log_log_energy_3000',
+                      ('morlet', {'dtype': dt}), 3000, 'log', 'log', 'energy'))
+        cases.append((f'gmw_{dt}_lp_log_nv16_7000', ('gmw', {'dtype': dt}),
+                      7000, 'log-piecewise:nv16', 'log', 'peak'))
+    for key, wav, N, scaletype, ssqtype, maprange in cases:
+        nv = 32
+        if scaletype.endswith(':nv16'):
+            scaletype, nv = scaletype.split(':')[0], 16
+        w = sq.Wavelet(wav)
+        dt = w.dtype
+        s64 = process_scales(scaletype, N, w, nv=nv)
+        s_dt = s64.astype(dt)                  # _cwt.py:274-275
+        s_p, cwt_st, na, nvs = process_scales(s_dt, N, get_params=True)
+        st = ssqtype or cwt_st
+        f = _compute_associated_frequencies(s_p, N, w, st, maprange,
+                                            was_padded=True, dt=1,
+                                            transform='cwt')
+        if st.startswith('log'):
+            _, params = _get_params_find_closest_log(f)
+            params = {k: float(v) for k, v in params.items()}
+        else:
+            params = dict(vmin=float(f[0]), dv=float(f[1] - f[0]))
+        if cwt_st.startswith('log'):
+            const = np.log(2) / nvs
+        else:
+            const = ((s_p[1] - s_p[0]) / s_p).squeeze()
+        grids[key + '/scales64'] = s64.ravel()
+        grids[key + '/scales'] = s_dt.ravel()
+        grids[key + '/ssq_freqs'] = np.asarray(f)        # ascending, pre-flip
+        grids[key + '/nv'] = np.asarray(nvs, dtype=np.float64).ravel() if nvs is not None else np.zeros(0)
+        grids[key + '/const'] = np.asarray(const, dtype=np.float64).ravel()
+        grids[key + '/meta'] = np.array([N, na, nv], dtype=np.int64)
+        grids[key + '/info'] = np.array([scaletype, st, maprange, dt, wav[0],
+                                         repr(sorted(wav[1].items()))])
+        for k, v in params.items():
+            grids[key + '/p_' + k] = np.array(v, dtype=np.float64)
+        print('grid', key, na, flush=True)
+    np.savez_compressed(os.path.join(OUT, 'grids.npz'), **grids)
+
+    # ------------------------------------------------------------- e2e cwt
+    e2e = {}
+    N = 100
+    x = chirp_tone_noise(N, seed=1)
+    e2e['x'] = x
+    xb = np.stack([chirp_tone_noise(N, seed=2), chirp_tone_noise(N, seed=3)])
+    e2e['xb'] = xb
+    runs = [
+        ('gmw64_lp', ('gmw', {'dtype': 'float64'}), dict(), True),
+        ('morlet64_lp_log', ('morlet', {'mu': 6., 'dtype': 'float64'}),
+         dict(ssq_freqs='log'), True),
+        ('gmw64_log', ('gmw', {'dtype': 'float64'}), dict(scales='log'), False),
+        ('gmw64_lin', ('gmw', {'dtype': 'float64'}),
+         dict(scales='linear'), False),
+        ('gmw64_lp_noflip', ('gmw', {'dtype': 'float64'}),
+         dict(flipud=False), False),
+        ('gmw32_lp', ('gmw', {'dtype': 'float32'}), dict(), False),
+    ]
+    for key, wav, kw, full in runs:
+        out = sq.ssq_cwt(x.astype(np.float64), sq.Wavelet(wav), get_w=True,
+                         get_dWx=True, **kw)
+        Tx, Wx, f, s, w, dWx = out
+        # full precision for the two main cases, complex64 storage elsewhere
+        cast = (lambda a: a) if full else (lambda a: a.astype(np.complex64))
+        e2e[key + '/Tx'] = cast(Tx)
+        e2e[key + '/Wx'] = cast(Wx)
+        e2e[key + '/ssq_freqs'] = f
+        e2e[key + '/scales'] = s
+        if full:
+            e2e[key + '/w'] = w
+            e2e[key + '/dWx'] = dWx
+        # the fused (get_w=False) path gives the same Tx in the reference
+        Tx2 = sq.ssq_cwt(x.astype(np.float64), sq.Wavelet(wav), **kw)[0]
+        e2e[key + '/Tx_fused_maxdiff'] = np.abs(Tx2 - Tx).max()
+        print('e2e', key, Tx.shape, Tx.dtype, flush=True)
+    # batched
+    Tx, Wx, f, s = sq.ssq_cwt(xb.astype(np.float64),
+                              sq.Wavelet(('gmw', {'dtype': 'float64'})))
+    e2e['batch/Tx'] = Tx.astype(np.complex64)
+    e2e['batch/Wx'] = Wx.astype(np.complex64)
+    e2e['batch/ssq_freqs'], e2e['batch/scales'] = f, s
+    # cwt alone, derivative
+    Wx, s, dWx = sq.cwt(x.astype(np.float64),
+                        sq.Wavelet(('morlet', {'mu': 6., 'dtype': 'float64'})),
+                        derivative=True)
+    e2e['cwt_morlet64/Wx'] = Wx.astype(np.complex64)
+    e2e['cwt_morlet64/dWx'] = dWx.astype(np.complex64)
+    e2e['cwt_morlet64/scales'] = s
+    np.savez_compressed(os.path.join(OUT, 'e2e_cwt.npz'), **e2e)
+
+    # ------------------------------------------------------------ e2e stft
+    st = {'x': x}
+    for key, kw in [('hann64_64', dict(window='hann', n_fft=64, dtype='float64')),
+                    ('dpss_default', dict(dtype='float64')),
+                    ('hann32_fs', dict(window='hann', n_fft=32, fs=2.,
+                                       flipud=True, dtype='float64')),
+                    ('hann64_32', dict(window='hann', n_fft=64, dtype='float32'))]:
+        Tx, Sx, f, Sfs, w, dSx = sq.ssq_stft(x.astype(np.float64), get_w=True,
+                                             get_dWx=True, **kw)
+        Tx2 = sq.ssq_stft(x.astype(np.float64), **kw)[0]
+        st[key + '/Tx'], st[key + '/Sx'] = Tx, Sx
+        st[key + '/ssq_freqs'], st[key + '/Sfs'] = f, Sfs
+        st[key + '/w'], st[key + '/dSx'] = w, dSx
+        st[key + '/Tx_fused_maxdiff'] = np.abs(Tx2 - Tx).max()
+        print('stft', key, Tx.shape, Tx.dtype, flush=True)
+    Sx, dSx = sq.stft(x.astype(np.float64), 'hann', n_fft=64, hop_len=3,
+                      derivative=True, dtype='float64')
+    st['hop3/Sx'], st['hop3/dSx'] = Sx, dSx
+    np.savez_compressed(os.path.join(OUT, 'e2e_stft.npz'), **st)
+
+    # ------------------------------------------------------------- windows
+    win = {}
+    for name, n_fft, wl in [('hann', 512, 512), ('hann', 598, 598),
+                            (None, 598, 598), (None, 64, 64), ('hann', 64, 48)]:
+        for dt in ('float32', 'float64'):
+            w, dw = get_window(name, wl, n_fft, derivative=True, dtype=dt)
+            key = f'{name}_{n_fft}_{wl}_{dt}'
+            win[key + '/window'], win[key + '/diff_window'] = w, dw
+    np.savez_compressed(os.path.join(OUT, 'windows.npz'), **win)
+
+    # ------------------------------------------------------------- squeeze
+    # tests/fft_test.py:236-377 setups, made smaller
+    def _make_ssq_freqs(M, scaletype):
+        if scaletype == 'log-piecewise':
+            sf = np.logspace(0, np.log10(M), 2*M)
+            sf1 = sf[:M//2]
+            sf2 = sf[M//2 + 3 - 1::3]
+            return np.hstack([sf1, sf2])
+        elif scaletype == 'log':
+            return np.logspace(0, np.log10(M), M)
+        return np.linspace(0, M, M)
+
+    sqz = {}
+    np.random.seed(0)
+    Wx = np.random.randn(40, 128) * (1 + 2j)
+    dWx = np.random.randn(40, 128) * (2 - 1j)
+    w = np.abs(np.random.randn(40, 128))
+    w *= (2 * len(Wx) / w.max())
+    const = np.random.rand(40) + 0.5
+    sqz['Wx'], sqz['dWx'], sqz['w'], sqz['const'] = Wx, dWx, w, const
+    for stype in ('log-piecewise', 'log', 'linear'):
+        f = _make_ssq_freqs(len(Wx), stype)
+        sqz[stype + '/ssq_freqs'] = f
+        logscale = stype.startswith('log')
+        for flip in (False, True):
+            key = f'{stype}/{int(flip)}'
+            sqz[key + '/onfly'] = indexed_sum_onfly(Wx, w, f, const, logscale,
+                                                    flip, parallel=False)
+            # ssq_freqs scaled so the phase of the random data lands inside
+            sqz[key + '/fast'] = ssqueeze_fast(Wx, dWx, f / len(Wx) / 4, const,
+                                               logscale, flip, gamma=1e-2,
+                                               parallel=False)
+    Sfs = np.linspace(0, .5, len(Wx))
+    sqz['stft/Sfs'] = Sfs
+    sqz['stft/fast'] = ssqueeze_fast(Wx, dWx, Sfs, const, False, False,
+                                     gamma=1e-2, Sfs=Sfs, parallel=False)
+    sqz['phase_cwt'] = phase_cwt_cpu(Wx, dWx, 1e-2, parallel=False)
+    sqz['phase_stft'] = phase_stft_cpu(Wx, dWx, Sfs, 1e-2, parallel=False)
+    np.savez_compressed(os.path.join(OUT, 'squeeze.npz'), **sqz)
+
+    # ------------------------------------------------------------------ C1
+    N = 10000
+    x = chirp_tone_noise(N, seed=0)
+    wav = sq.Wavelet(('morlet', {'mu': 6., 'dtype': 'float64'}))
+    Tx, Wx, f, s, w = sq.ssq_cwt(x.astype(np.float64), wav, ssq_freqs='log',
+                                 get_w=True)
+    cols = np.linspace(0, N - 1, 24).astype(np.int64)
+    c1 = dict(x=x, ssq_freqs=f, scales=s, cols=cols,
+              Tx_rownorm=np.linalg.norm(Tx, axis=1),
+              Wx_rownorm=np.linalg.norm(Wx, axis=1),
+              Tx_cols=Tx[:, cols], Wx_cols=Wx[:, cols], w_cols=w[:, cols],
+              Tx_norm=np.linalg.norm(Tx), Wx_norm=np.linalg.norm(Wx))
+    np.savez_compressed(os.path.join(OUT, 'c1.npz'), **c1)
+    print('c1', Tx.shape, flush=True)
+
+
+if __name__ == '__main__':
+    main()
