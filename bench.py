@@ -1,0 +1,142 @@
+"""Throughput of ssq_cwt on one MI355X.
+
+Workload (default): one float32 signal, N = 160,000 samples, linear chirp
+0.02->0.2 cycles/sample + 0.5 tone at 0.3 + 0.1 Gaussian noise (seed 0),
+GMW(gamma=3, beta=60) bandpass wavelet, nv=32 log-piecewise scales
+(na = 293), reflect padding to L = 262,144, log-piecewise ssq_freqs,
+peak maprange.  A step is one ssq_cwt call on a signal already in HBM
+producing Tx and Wx in HBM.
+
+Metric: Gscale*samples/s = na * N / step time / 1e9.  Roofline: 16 B of
+algorithmic HBM traffic per scale*sample (Wx and Tx written once as
+complex64) + 4 B per sample of x, against the 8 TB/s HBM peak.
+CPU baseline: the reference's parallel CPU pipeline restated (scipy.fft on
+all host threads + the OpenMP port of the column-parallel squeeze loop) on
+the same workload, 1 warmup + 2 timed calls.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def chirp_tone_noise(N, seed):
+    n = np.arange(N, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    x = (np.cos(2 * np.pi * (0.02 * n + 0.09 * n**2 / N))
+         + 0.5 * np.cos(2 * np.pi * 0.3 * n) + 0.1 * rng.standard_normal(N))
+    return x.astype(np.float32)
+
+
+def cpu_baseline(x, g, reps=2):
+    """The reference's CPU path (float32, parallel) on the same workload."""
+    from oracle import cpu_baseline as cb
+    from oracle import ssq_oracle as O
+    params = O.bin_params(g['ssq_freqs'], True)
+    runner = cb.SsqCwtCPU(len(x), g['scales64'], g['const'], params, g['na'] - 1,
+                          flipud=True)
+    runner(x)                                   # warmup (+ page-in)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        runner(x)
+    dt = (time.perf_counter() - t0) / reps
+    return dict(value=g['na'] * len(x) / dt / 1e9, unit='Gscale*samples/s',
+                cores=os.cpu_count(), kind='port',
+                sample=f"full workload (N={len(x)}, na={g['na']}), 1 warmup + "
+                       f"{reps} timed calls, {dt:.2f} s/call; scipy.fft workers="
+                       f"{runner.workers}, OpenMP squeeze",
+                s_per_call=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--N', type=int, default=160000)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only (this build runs the transform on a "
+              "single MI355X; --gpus %d is not supported)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+
+    N = args.N
+    x = chirp_tone_noise(N, 0)
+    wav = sq.Wavelet('gmw')
+    g = ssq_cwt_grid(N, wav)
+    na = g['na']
+    dev = torch.device('cuda', 0)
+    x_dev = torch.as_tensor(x, device=dev)
+
+    def step():
+        return sq.ssq_cwt(x_dev, wav)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        out = step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_gpu = ev0.elapsed_time(ev1) / args.steps
+    ms_step = wall * 1e3 / args.steps
+    units = na * N
+    value = units / (ms_step * 1e-3) / 1e9
+    alg_bytes = 4 * N + 16 * units
+    achieved = alg_bytes / (ms_gpu * 1e-3) / 1e9
+
+    res = {
+        "metric": "ssq_cwt Gscale*samples/s (N=160k, ~300 scales, fp32)",
+        "value": round(value, 3),
+        "unit": "Gscale*samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "ssq_cwt, 1 signal, N=%d, GMW(3,60), nv=32 "
+                               "log-piecewise (na=%d), reflect pad L=%d, "
+                               "log-piecewise ssq_freqs" % (N, na, 2 * 2**int(np.round(np.log2(N)))),
+                   "batch": 1, "N": N, "na": na},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "ssq_cwt step (all kernels of one call, HIP events "
+                               "on the launch stream)",
+                     "ms_gpu_per_step": round(ms_gpu, 4),
+                     "alg_bytes_per_step": alg_bytes},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(x, g)
+        res["cpu_baseline"] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind',
+                                                  'sample')}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,147 @@
+/* ssqk -- synchrosqueezing transforms on AMD Instinct MI355X (gfx950).
+ *
+ * Plain C ABI: device pointers, sizes and a HIP stream passed as void*.
+ * Every entry point returns SSQK_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is available from
+ * ssqk_last_error().  Outputs are caller-allocated device buffers; the
+ * library keeps only per-device twiddle tables and the plans it creates.
+ *
+ * Each entry point names the reference (ssqueezepy 0.6.6) interface it
+ * replaces; paths are relative to the reference package root.
+ */
+#ifndef SSQK_H_
+#define SSQK_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSQK_OK 0
+#define SSQK_ERR_ARG (-1)
+#define SSQK_ERR_HIP (-2)
+#define SSQK_ERR_UNSUPPORTED (-3)
+
+#define SSQK_WAVELET_GMW 0
+#define SSQK_WAVELET_MORLET 1
+
+#define SSQK_PAD_REFLECT 0
+#define SSQK_PAD_ZERO 1
+#define SSQK_PAD_SYMMETRIC 2
+#define SSQK_PAD_REPLICATE 3
+#define SSQK_PAD_WRAP 4
+
+#define SSQK_BIN_LINEAR 0
+#define SSQK_BIN_LOG 1
+#define SSQK_BIN_LOG_PIECEWISE 2
+
+/* Squeeze-bin parameters as derived by _process_ssq_params /
+ * _get_params_find_closest_log (ssqueezepy/algos.py:44-123, 356-374).
+ * LINEAR: v0 = vmin, d0 = dv.  LOG: v0 = vlmin, d0 = dvl.
+ * LOG_PIECEWISE: v0 = vlmin0, d0 = dvl0, v1 = vlmin1, d1 = dvl1, idx1.
+ * omax = number of output rows - 1; flipud maps k -> omax - k. */
+typedef struct {
+  int32_t type;
+  int32_t flipud;
+  int64_t omax;
+  int64_t idx1;
+  double v0, d0, v1, d1;
+} ssqk_bin_params;
+
+typedef struct ssqk_cwt_plan_s* ssqk_cwt_plan;
+
+const char* ssqk_last_error(void);
+int ssqk_version(void);
+
+/* ------------------------------------------------------------- wavelets
+ * psih(scale_i * xi_k) for k in [0, L), i in [0, na): the wavelet sampled on
+ * the L-point DFT grid, optionally with the Nyquist bin halved.
+ * Replaces Wavelet.__call__(scale=..., N=..., nohalf=...) and Wavelet.Psih
+ * (wavelets.py:62-95, 135-160) with the Morlet / GMW-L1 functions
+ * (wavelets.py:498-527, _gmw.py:187-219).
+ * params (host): GMW {gamma, beta}; Morlet {mu}.  scales: device float64
+ * [na].  out: device float32 [na, L]. */
+int ssqk_psih(int wavelet_kind, const double* params, const double* scales,
+              int64_t na, int64_t L, int nohalf, float* out, void* stream);
+
+/* ------------------------------------------------------------------ CWT
+ * A plan holds the per-scale frequency bands, the task tables and the
+ * launch partition of the CWT of length-N signals padded to L (a power of
+ * two) with left pad n1.  Replaces the setup part of cwt()
+ * (_cwt.py:246-291): padding parameters, wavelet grid and Psih cache.
+ * scales: host float64 [na].  band_rel: psih values below band_rel * peak
+ * are treated as zero (1e-10 keeps results within float32 rounding). */
+int ssqk_cwt_plan_create(ssqk_cwt_plan* plan, int64_t N, int64_t L,
+                         int64_t n1, int pad_type, int wavelet_kind,
+                         const double* wavelet_params, const double* scales,
+                         int64_t na, double dt, double band_rel);
+int ssqk_cwt_plan_destroy(ssqk_cwt_plan plan);
+/* Device scratch bytes ssqk_cwt_exec needs for `batch` signals. */
+int ssqk_cwt_workspace_size(ssqk_cwt_plan plan, int64_t batch, size_t* bytes);
+
+/* Forward CWT of `batch` float32 signals x [batch, N] (device):
+ *   Wx  [batch, na, N] complex64   (always written)
+ *   dWx [batch, na, N] complex64   (optional, NULL to skip)
+ *   w   [batch, na, N] float32     phase transform, inf where |Wx| < gamma
+ *                                  (optional)
+ *   bins [batch, na, N] uint16     squeeze row of every point, 0xFFFF where
+ *                                  |Wx| <= gamma (optional; needs bp)
+ * Replaces cwt(..., derivative=True) (_cwt.py:167-177, 246-320: pad, fft,
+ * psih * xh, ifft, unpad), phase_cwt (_ssq_cwt.py:420-509, algos.py:706-781)
+ * and the bin search of ssqueeze_fast (algos.py:859-953).
+ * gamma_strict: 1 bins points with |Wx| > gamma (ssqueeze_fast), 0 bins
+ * points with |Wx| >= gamma (phase_cwt + indexed_sum_onfly). */
+int ssqk_cwt_exec(ssqk_cwt_plan plan, const float* x, int64_t batch,
+                  void* Wx, void* dWx, float* w, uint16_t* bins,
+                  const ssqk_bin_params* bp, double gamma, int gamma_strict,
+                  void* workspace, void* stream);
+
+/* ------------------------------------------------------------- squeeze
+ * Tx[b, k, j] = sum over i (ascending) of Wx[b, i, j] * cst[i] for
+ * bins[b, i, j] == k.  Deterministic: each column's sum is formed in one
+ * workgroup in a fixed order.  Replaces the accumulation of ssqueeze_fast /
+ * indexed_sum_onfly (algos.py:126-250, 859-984) and the Tx allocation of
+ * ssqueeze (ssqueezing.py:205).  Wx, Tx complex64; cst float32 [na]. */
+int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* cst,
+                 int64_t batch, int64_t na, int64_t nf, int64_t N, void* Tx,
+                 void* stream);
+
+/* bins from (Wx, dWx) with the fused phase transform; Sfs (float32 [na])
+ * non-NULL selects the STFT form |Sfs[i] - Im(dWx/Wx)/2pi| (algos.py:956-984).
+ * Replaces the bin/threshold part of ssqueeze_fast (algos.py:126-150). */
+int ssqk_bins_from_dwx(const void* Wx, const void* dWx, const float* Sfs,
+                       int64_t rows, int64_t N, const ssqk_bin_params* bp,
+                       double gamma, uint16_t* bins, void* stream);
+/* bins from a given phase transform w (inf = skipped); replaces the bin
+ * search of indexed_sum_onfly (algos.py:153-250). */
+int ssqk_bins_from_w(const float* w, int64_t rows, int64_t N,
+                     const ssqk_bin_params* bp, uint16_t* bins, void* stream);
+/* Phase transforms: w = |Im(dWx/Wx)| / 2pi, or |Sfs[i] - Im(dWx/Wx)/2pi|
+ * with Sfs non-NULL; inf where |Wx| < gamma.  Replaces phase_cwt_gpu /
+ * phase_stft_gpu (algos.py:743-781, 818-856). */
+int ssqk_phase(const void* Wx, const void* dWx, const float* Sfs,
+               int64_t rows, int64_t N, double gamma, float* w, void* stream);
+
+/* ----------------------------------------------------------------- STFT
+ * Sx / dSx [batch, n_fft/2+1, n_frames] complex64 of `batch` float32
+ * signals (device [batch, N]), reflect/zero padded to N + n_fft - 1, frames
+ * every `hop` samples, optionally modulated (frame ifftshift), window and
+ * diff_window given on the device (float32 [n_fft]); n_fft a power of two
+ * in [2, 8192].  dSx may be NULL.  Optionally bins [batch, n_fft/2+1,
+ * n_frames] for the STFT squeeze (needs dSx, bp, gamma and Sfs, the float32
+ * row frequencies).  Replaces stft() (_stft.py:13-181: padsignal, buffer,
+ * window multiply, rfft) and the bin part of the ssq_stft kernel
+ * (algos.py:1130-1167). */
+int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t n_fft,
+                   int64_t hop, int pad_type, int modulated, const float* window,
+                   const float* diff_window, double fs, void* Sx, void* dSx,
+                   const float* Sfs, uint16_t* bins, const ssqk_bin_params* bp,
+                   double gamma, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* SSQK_H_ */

@@ -1,0 +1,24 @@
+"""ssqueezepy_amd -- synchrosqueezing transforms on AMD MI355X.
+
+Drop-in for the forward transforms of ssqueezepy: ``ssq_cwt``, ``ssq_stft``,
+``cwt``, ``stft`` keep the reference signatures and return tuples; the
+arithmetic runs in hand-written HIP kernels (libssqk.so, C ABI in
+include/ssqk.h).  Host-side grid construction (scales, ssq_freqs, windows)
+follows the reference exactly.
+"""
+from .wavelets import Wavelet, center_frequency, morlet, gmw
+from ._scales import (process_scales, cwt_scalebounds, make_scales,
+                      infer_scaletype, logscale_transition_idx, nv_from_scales)
+from ._common import p2up, padsignal_np as padsignal
+from ._cwt import cwt
+from ._ssq_cwt import ssq_cwt, phase_cwt
+from ._stft import stft, get_window
+from ._ssq_stft import ssq_stft, phase_stft
+from .ssqueezing import ssqueeze
+from . import algos
+
+__version__ = '0.1.0'
+
+
+def wavs():
+    return Wavelet.SUPPORTED

@@ -1,0 +1,236 @@
+"""ctypes binding of libssqk.so (the HIP kernels behind the C ABI in
+include/ssqk.h).
+
+The library must be built (``python -m ssqueezepy_amd._build``); there is no
+fallback: every transform in this package runs on the GPU through it.
+"""
+import ctypes
+import os
+import threading
+from collections import OrderedDict
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libssqk.so')
+
+c_int, c_int32, c_int64 = ctypes.c_int, ctypes.c_int32, ctypes.c_int64
+c_double, c_void_p, c_size_t = ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+BIN_LINEAR, BIN_LOG, BIN_LOG_PIECEWISE = 0, 1, 2
+PAD_CODES = {'reflect': 0, 'zero': 1, 'symmetric': 2, 'replicate': 3, 'wrap': 4}
+NO_BIN = 0xFFFF
+
+
+class BinParams(ctypes.Structure):
+    _fields_ = [('type', c_int32), ('flipud', c_int32), ('omax', c_int64),
+                ('idx1', c_int64), ('v0', c_double), ('d0', c_double),
+                ('v1', c_double), ('d1', c_double)]
+
+
+_SIGS = {
+    'ssqk_last_error': (ctypes.c_char_p, []),
+    'ssqk_version': (c_int, []),
+    'ssqk_psih': (c_int, [c_int, c_double_p, c_void_p, c_int64, c_int64, c_int,
+                          c_void_p, c_void_p]),
+    'ssqk_cwt_plan_create': (c_int, [ctypes.POINTER(c_void_p), c_int64, c_int64,
+                                     c_int64, c_int, c_int, c_double_p, c_double_p,
+                                     c_int64, c_double, c_double]),
+    'ssqk_cwt_plan_destroy': (c_int, [c_void_p]),
+    'ssqk_cwt_workspace_size': (c_int, [c_void_p, c_int64,
+                                        ctypes.POINTER(c_size_t)]),
+    'ssqk_cwt_exec': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                              c_void_p, c_void_p, ctypes.POINTER(BinParams),
+                              c_double, c_int, c_void_p, c_void_p]),
+    'ssqk_squeeze': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                             c_int64, c_int64, c_void_p, c_void_p]),
+    'ssqk_bins_from_dwx': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                   ctypes.POINTER(BinParams), c_double, c_void_p,
+                                   c_void_p]),
+    'ssqk_bins_from_w': (c_int, [c_void_p, c_int64, c_int64,
+                                 ctypes.POINTER(BinParams), c_void_p, c_void_p]),
+    'ssqk_phase': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                           c_double, c_void_p, c_void_p]),
+    'ssqk_stft_exec': (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int,
+                               c_int, c_void_p, c_void_p, c_double, c_void_p,
+                               c_void_p, c_void_p, c_void_p,
+                               ctypes.POINTER(BinParams), c_double, c_void_p]),
+}
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libssqk.so and declare the signatures; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    "libssqk.so is not built (%s); run `python -m "
+                    "ssqueezepy_amd._build` first" % LIB_PATH)
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().ssqk_last_error().decode(errors='replace')
+        if rc == -3:
+            raise NotImplementedError("ssqk: " + msg)
+        raise RuntimeError("ssqk error %d: %s" % (rc, msg))
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def bin_params(ptype, flipud, omax, params):
+    """Pack the squeeze-bin parameters of _process_ssq_params
+    (algos.py:44-123) into the C struct."""
+    bp = BinParams()
+    bp.type = ptype
+    bp.flipud = int(bool(flipud))
+    bp.omax = int(omax)
+    if ptype == BIN_LINEAR:
+        bp.v0, bp.d0 = params['vmin'], params['dv']
+    elif ptype == BIN_LOG:
+        bp.v0, bp.d0 = params['vlmin'], params['dvl']
+    else:
+        bp.v0, bp.d0 = params['vlmin0'], params['dvl0']
+        bp.v1, bp.d1 = params['vlmin1'], params['dvl1']
+        bp.idx1 = int(params['idx1'])
+    return bp
+
+
+# ----------------------------------------------------------------- plans
+class CwtPlan:
+    """Owner of an ssqk_cwt_plan (per-scale bands and task tables)."""
+
+    def __init__(self, device, N, L, n1, pad, kind, params, scales64, dt,
+                 band_rel=1e-10):
+        import torch
+        lib = load()
+        self.device = device
+        self.N, self.L, self.n1, self.na = int(N), int(L), int(n1), len(scales64)
+        handle = c_void_p()
+        p = (c_double * 2)(*params)
+        s = np.ascontiguousarray(scales64, dtype=np.float64)
+        with torch.cuda.device(device):
+            check(lib.ssqk_cwt_plan_create(
+                ctypes.byref(handle), self.N, self.L, self.n1, PAD_CODES[pad],
+                kind, p, s.ctypes.data_as(c_double_p), self.na, float(dt),
+                float(band_rel)))
+        self._h = handle
+
+    def workspace_bytes(self, batch):
+        n = c_size_t()
+        check(load().ssqk_cwt_workspace_size(self._h, int(batch), ctypes.byref(n)))
+        return n.value
+
+    def exec(self, x, Wx, dWx=None, w=None, bins=None, bp=None, gamma=0.,
+             strict=True, workspace=None):
+        import torch
+        batch = 1 if x.ndim == 1 else x.shape[0]
+        if workspace is None:
+            workspace = torch.empty(self.workspace_bytes(batch), dtype=torch.uint8,
+                                    device=x.device)
+        check(load().ssqk_cwt_exec(
+            self._h, _ptr(x), batch, _ptr(Wx), _ptr(dWx), _ptr(w), _ptr(bins),
+            ctypes.byref(bp) if bp is not None else None, float(gamma),
+            int(bool(strict)), _ptr(workspace), _stream(x.device)))
+        return workspace
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and _lib is not None:
+            try:
+                _lib.ssqk_cwt_plan_destroy(h)
+            except Exception:
+                pass
+
+
+_plans = OrderedDict()
+
+
+def cwt_plan(device, N, L, n1, pad, kind, params, scales64, dt):
+    """Cached CwtPlan for (device, sizes, wavelet, scales, dt)."""
+    key = (str(device), N, L, n1, pad, kind, tuple(params),
+           np.asarray(scales64, dtype=np.float64).tobytes(), float(dt))
+    plan = _plans.get(key)
+    if plan is None:
+        plan = CwtPlan(device, N, L, n1, pad, kind, params, scales64, dt)
+        _plans[key] = plan
+        while len(_plans) > 32:
+            _plans.popitem(last=False)
+    else:
+        _plans.move_to_end(key)
+    return plan
+
+
+# ---------------------------------------------------------- thin wrappers
+def squeeze(Wx, bins, cst, nf, Tx):
+    batch = 1 if Wx.ndim == 2 else Wx.shape[0]
+    na, N = Wx.shape[-2], Wx.shape[-1]
+    check(load().ssqk_squeeze(_ptr(Wx), _ptr(bins), _ptr(cst), batch, na, nf, N,
+                              _ptr(Tx), _stream(Wx.device)))
+
+
+def bins_from_dwx(Wx, dWx, Sfs, bp, gamma, bins):
+    rows = int(np.prod(Wx.shape[:-1]))
+    check(load().ssqk_bins_from_dwx(_ptr(Wx), _ptr(dWx), _ptr(Sfs), rows,
+                                    Wx.shape[-1], ctypes.byref(bp), float(gamma),
+                                    _ptr(bins), _stream(Wx.device)))
+
+
+def bins_from_w(w, bp, bins):
+    rows = int(np.prod(w.shape[:-1]))
+    check(load().ssqk_bins_from_w(_ptr(w), rows, w.shape[-1], ctypes.byref(bp),
+                                  _ptr(bins), _stream(w.device)))
+
+
+def phase(Wx, dWx, Sfs, gamma, w):
+    rows = int(np.prod(Wx.shape[:-1]))
+    check(load().ssqk_phase(_ptr(Wx), _ptr(dWx), _ptr(Sfs), rows, Wx.shape[-1],
+                            float(gamma), _ptr(w), _stream(Wx.device)))
+
+
+def psih(wavelet, scales64, N, nohalf, out):
+    import torch
+    kind = wavelet.kind
+    p = wavelet.fn.params64
+    params = (c_double * 2)(*((p['gamma'], p['beta']) if kind == 0 else
+                              (p['mu'], 0.)))
+    s = torch.as_tensor(np.asarray(scales64, dtype=np.float64), device=out.device)
+    if out.dtype != torch.float32:
+        raise NotImplementedError("device psih is generated in float32")
+    check(load().ssqk_psih(kind, params, _ptr(s), len(scales64), int(N),
+                           int(bool(nohalf)), _ptr(out), _stream(out.device)))
+    return out
+
+
+def stft_exec(x, n_fft, hop, pad, modulated, window, diff_window, fs, Sx, dSx,
+              Sfs=None, bins=None, bp=None, gamma=0.):
+    batch = 1 if x.ndim == 1 else x.shape[0]
+    check(load().ssqk_stft_exec(
+        _ptr(x), batch, x.shape[-1], int(n_fft), int(hop), PAD_CODES[pad],
+        int(bool(modulated)), _ptr(window), _ptr(diff_window), float(fs),
+        _ptr(Sx), _ptr(dSx), _ptr(Sfs), _ptr(bins),
+        ctypes.byref(bp) if bp is not None else None, float(gamma),
+        _stream(x.device)))

@@ -1,0 +1,199 @@
+"""Synchrosqueezed CWT on the GPU (ssq_cwt of _ssq_cwt.py:12-310, phase_cwt
+of :420-509).
+
+One call to ``ssqk_cwt_exec`` produces Wx and, per point, the squeeze row of
+its phase transform (dWx and w only when asked for); ``ssqk_squeeze`` then
+sums every column in ascending scale order.
+"""
+import numpy as np
+
+from . import _lib
+from ._common import EPS32, EPS64, _process_fs_and_t
+from ._cwt import _nan_checks, run_cwt
+from ._scales import process_scales
+from ._tensors import as_device_tensor, signal_to_device, to_output
+from .ssqueezing import (_apply_squeezing, _check_ssqueezing_args,
+                         bin_params_from_freqs, compute_associated_frequencies,
+                         cwt_const)
+from .wavelets import Wavelet
+
+__all__ = ['ssq_cwt', 'phase_cwt', 'ssq_cwt_grid']
+
+
+def _as_float64(wavelet):
+    """The same wavelet with float64 host evaluation."""
+    if wavelet.dtype == 'float64':
+        return wavelet
+    opts = dict(wavelet.config)
+    opts['dtype'] = 'float64'
+    return Wavelet((wavelet.name_short, opts), N=wavelet.N)
+
+
+_GRID_CACHE = {}
+
+
+def _key_part(v):
+    if isinstance(v, np.ndarray):
+        return ('nd', v.dtype.str, v.shape, v.tobytes())
+    if hasattr(v, 'detach'):
+        a = v.detach().cpu().numpy()
+        return ('nd', a.dtype.str, a.shape, a.tobytes())
+    if isinstance(v, Wavelet):
+        return ('wav', v.name_short, repr(sorted(v.config.items())))
+    if isinstance(v, (tuple, list)):
+        return tuple(_key_part(u) for u in v)
+    if isinstance(v, dict):
+        return tuple(sorted((k, _key_part(u)) for k, u in v.items()))
+    return v
+
+
+def ssq_cwt_grid(N, wavelet, scales='log-piecewise', nv=None, fs=None, t=None,
+                 ssq_freqs=None, maprange='peak', padtype='reflect',
+                 flipud=True):
+    """Memoized ``_ssq_cwt_grid``: the grid depends only on these arguments
+    (the reference's benchmark likewise precomputes scales and ssq_freqs)."""
+    try:
+        key = tuple(_key_part(v) for v in (N, wavelet, scales, nv, fs, t, ssq_freqs,
+                                           maprange, padtype, flipud))
+        hash(key)
+    except TypeError:
+        return _ssq_cwt_grid(N, wavelet, scales, nv, fs, t, ssq_freqs, maprange,
+                             padtype, flipud)
+    g = _GRID_CACHE.get(key)
+    if g is None:
+        g = _ssq_cwt_grid(N, wavelet, scales, nv, fs, t, ssq_freqs, maprange,
+                          padtype, flipud)
+        if len(_GRID_CACHE) > 64:
+            _GRID_CACHE.clear()
+        _GRID_CACHE[key] = g
+    return g
+
+
+def _ssq_cwt_grid(N, wavelet, scales='log-piecewise', nv=None, fs=None, t=None,
+                  ssq_freqs=None, maprange='peak', padtype='reflect',
+                  flipud=True):
+    """Host grid of ssq_cwt: (scales64 [na], scales in the wavelet dtype,
+    ssq_freqs ascending, squeeze const [na], bin parameters, cwt_scaletype).
+
+    Follows _ssq_cwt.py:190-289 and ssqueezing.py:160-214 evaluated with
+    float64 wavelet samples, i.e. the grid of the float64 reference: the
+    float32 reference's discrete center-frequency searches can land one
+    sample off at large N (2^20), moving its ssq_freqs by ~1e-6."""
+    wavelet = Wavelet._init_if_not_isinstance(wavelet, N=N)
+    wav64 = _as_float64(wavelet)
+    if nv is None and not isinstance(scales, np.ndarray):
+        nv = 32
+    dt, fs, _ = _process_fs_and_t(fs, t, N)
+    scales_arr, cwt_scaletype, na, _ = process_scales(scales, N, wav64, nv=nv,
+                                                      get_params=True)
+    scales64 = np.asarray(scales_arr, dtype=np.float64).ravel()
+    scales_dt = scales64.astype(wavelet.dtype)
+    # ssqueeze re-derives the scale type and nv from the scale array
+    scales_p, st64, _, nv64 = process_scales(scales64.reshape(-1, 1), N,
+                                             get_params=True)
+    if ssq_freqs is None:
+        ssq_freqs = st64
+    if isinstance(ssq_freqs, str):
+        ssq_scaletype = ssq_freqs
+        if ((maprange == 'maximal' or isinstance(maprange, tuple)) and
+                ssq_scaletype == 'log-piecewise'):
+            raise ValueError("can't have `ssq_scaletype = log-piecewise` or "
+                             "tuple with `maprange = 'maximal'` "
+                             "(got %s)" % str(maprange))
+        freqs = compute_associated_frequencies(
+            scales_p, N, wav64, ssq_scaletype, maprange,
+            was_padded=padtype is not None, dt=dt, transform='cwt')
+    else:
+        from ._scales import infer_scaletype
+        freqs = np.asarray(ssq_freqs.detach().cpu() if hasattr(ssq_freqs, 'detach')
+                           else ssq_freqs)
+        ssq_scaletype, _ = infer_scaletype(freqs)
+    const = cwt_const(scales_p, st64, nv64, na)
+    bp = bin_params_from_freqs(freqs, ssq_scaletype.startswith('log'), flipud,
+                               na - 1)
+    return dict(scales64=scales64, scales=scales_dt, ssq_freqs=freqs,
+                const=const, bp=bp, cwt_scaletype=st64, wavelet=wavelet,
+                dt=dt, fs=fs, na=na)
+
+
+def _device_const(g, dev):
+    """float32 squeeze weights on ``dev``, uploaded once per grid."""
+    import torch
+    cache = g.setdefault('_const_dev', {})
+    t = cache.get(str(dev))
+    if t is None:
+        t = torch.as_tensor(g['const'], dtype=torch.float32, device=dev)
+        cache[str(dev)] = t
+    return t
+
+
+def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
+            ssq_freqs=None, padtype='reflect', squeezing='sum', maprange='peak',
+            difftype='trig', difforder=None, gamma=None, vectorized=True,
+            preserve_transform=None, astensor=True, order=0, nan_checks=None,
+            patience=0, flipud=True, cache_wavelet=None,
+            get_w=False, get_dWx=False):
+    """Synchrosqueezed continuous wavelet transform (_ssq_cwt.py:12-310).
+
+    Returns (Tx, Wx, ssq_freqs, scales[, w][, dWx]) with Tx, Wx [na, N] (or
+    [B, na, N]) complex64 on the GPU, ssq_freqs float64 NumPy (descending
+    when flipped), scales float32.
+    """
+    import torch
+    if x.ndim == 2 and get_w:
+        raise NotImplementedError("`get_w=True` unsupported with batched input.")
+    if isinstance(order, (tuple, list, range)) or order > 0:
+        raise NotImplementedError("higher-order GMW ssq_cwt is not on the GPU path")
+    difforder = _check_ssqueezing_args(squeezing, maprange, wavelet, difftype,
+                                       difforder, get_w, transform='cwt')
+    x = _nan_checks(x, nan_checks)
+    N = x.shape[-1]
+    g = ssq_cwt_grid(N, wavelet, scales, nv, fs, t, ssq_freqs, maprange,
+                     padtype, flipud)
+    wavelet = g['wavelet']
+    x_dev = signal_to_device(x, wavelet.dtype)
+    dev = x_dev.device
+    na = g['na']
+    shape = (na, N) if x.ndim == 1 else (x.shape[0], na, N)
+    if gamma is None:
+        gamma = 10 * EPS32
+
+    Wx = torch.empty(shape, dtype=torch.complex64, device=dev)
+    dWx = torch.empty_like(Wx) if get_dWx else None
+    w = torch.empty(shape, dtype=torch.float32, device=dev) if get_w else None
+    bins = torch.empty(shape, dtype=torch.int16, device=dev)
+    # get_w: phase_cwt then indexed_sum_onfly (w = inf where |Wx| < gamma);
+    # otherwise the fused ssqueeze_fast test |Wx| > gamma
+    run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx, w=w,
+            bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w)
+    Wsq = _apply_squeezing(Wx, squeezing).contiguous()
+    Tx = torch.empty_like(Wx)
+    _lib.squeeze(Wsq, bins, _device_const(g, dev), na, Tx)
+
+    ssq_freqs = g['ssq_freqs'][::-1]          # ssqueezing.py:216-222
+    scales_out = torch.as_tensor(g['scales'], device=dev, dtype=torch.float32) \
+        if astensor else g['scales'].copy()
+    Tx, Wx, w, dWx, scales_out = [to_output(v, astensor)
+                                  for v in (Tx, Wx, w, dWx, scales_out)]
+    if get_w and get_dWx:
+        return Tx, Wx, ssq_freqs, scales_out, w, dWx
+    elif get_w:
+        return Tx, Wx, ssq_freqs, scales_out, w
+    elif get_dWx:
+        return Tx, Wx, ssq_freqs, scales_out, dWx
+    return Tx, Wx, ssq_freqs, scales_out
+
+
+def phase_cwt(Wx, dWx, difftype='trig', gamma=None, parallel=None):
+    """w = |Im(dWx / Wx)| / 2pi, inf where |Wx| < gamma
+    (_ssq_cwt.py:420-509, algos.py:706-781); default gamma = sqrt(eps32)."""
+    import torch
+    if difftype != 'trig':
+        raise ValueError("`difftype != 'trig'` unsupported with tensor inputs.")
+    Wx = as_device_tensor(Wx, complex_=True)
+    dWx = as_device_tensor(dWx, like=Wx, complex_=True)
+    if gamma is None:
+        gamma = np.sqrt(EPS32)
+    w = torch.empty(Wx.shape, dtype=torch.float32, device=Wx.device)
+    _lib.phase(Wx, dWx, None, gamma, w)
+    return w

@@ -1,0 +1,102 @@
+"""Synchrosqueezed STFT on the GPU (ssq_stft of _ssq_stft.py:13-136,
+phase_stft of :201-246, _make_Sfs of :249-257).
+
+``ssqk_stft_exec`` produces Sx and the squeeze row of every point (the
+phase transform |Sfs - Im(dSx/Sx)/2pi| is formed in registers and never
+stored unless asked for); ``ssqk_squeeze`` sums each frame's column.
+"""
+import numpy as np
+
+from . import _lib
+from ._common import EPS32, _process_fs_and_t
+from ._scales import infer_scaletype
+from ._stft import stft_setup
+from ._tensors import as_device_tensor, signal_to_device, to_output
+from .ssqueezing import _apply_squeezing, _check_ssqueezing_args, bin_params_from_freqs
+
+__all__ = ['ssq_stft', 'phase_stft']
+
+
+def _make_Sfs(n_rows, fs, dtype='float32'):
+    return np.linspace(0, .5 * fs, n_rows, dtype=dtype)
+
+
+def ssq_stft(x, window=None, n_fft=None, win_len=None, hop_len=1, fs=None, t=None,
+             modulated=True, ssq_freqs=None, padtype='reflect', squeezing='sum',
+             gamma=None, preserve_transform=None, dtype=None, astensor=True,
+             flipud=False, get_w=False, get_dWx=False):
+    """Synchrosqueezed STFT; returns (Tx, Sx, ssq_freqs, Sfs[, w][, dSx])
+    (_ssq_stft.py:13-136)."""
+    import torch
+    if x.ndim == 2 and get_w:
+        raise NotImplementedError("`get_w=True` unsupported with batched input.")
+    _, fs, _ = _process_fs_and_t(fs, t, x.shape[-1])
+    _check_ssqueezing_args(squeezing)
+    if (isinstance(ssq_freqs, np.ndarray) and
+            infer_scaletype(ssq_freqs)[0] != 'linear'):
+        raise ValueError("`ssq_freqs` must be linearly distributed "
+                         "for `ssq_stft`")
+    N, fs, n_fft, window, diff_window, dtype = stft_setup(
+        x, window, n_fft, win_len, hop_len, fs, None, padtype, dtype)
+    x_dev = signal_to_device(x, dtype)
+    dev = x_dev.device
+    n_frames = (N - 1) // hop_len + 1
+    rows = n_fft // 2 + 1
+    shape = (rows, n_frames) if x.ndim == 1 else (x.shape[0], rows, n_frames)
+    Sfs = _make_Sfs(rows, fs, dtype)
+    if gamma is None:
+        gamma = 10 * EPS32
+    if ssq_freqs is None:
+        ssq_freqs = Sfs
+    ssq_freqs = np.asarray(ssq_freqs)
+    # ssqueeze(transform='stft'): linear bins on ssq_freqs, const = df
+    bp = bin_params_from_freqs(ssq_freqs, False, flipud, rows - 1)
+    const = float(ssq_freqs[1] - ssq_freqs[0])
+
+    Sx = torch.empty(shape, dtype=torch.complex64, device=dev)
+    dSx = torch.empty_like(Sx)
+    bins = torch.empty(shape, dtype=torch.int16, device=dev)
+    win = torch.as_tensor(window, device=dev)
+    dwin = torch.as_tensor(diff_window, device=dev)
+    Sfs_t = torch.as_tensor(Sfs, dtype=torch.float32, device=dev)
+    w = None
+    if get_w:
+        _lib.stft_exec(x_dev, n_fft, hop_len, padtype, modulated, win, dwin, fs,
+                       Sx, dSx)
+        w = torch.empty(shape, dtype=torch.float32, device=dev)
+        _lib.phase(Sx, dSx, Sfs_t, gamma, w)
+        _lib.bins_from_w(w, bp, bins)
+    else:
+        _lib.stft_exec(x_dev, n_fft, hop_len, padtype, modulated, win, dwin, fs,
+                       Sx, dSx, Sfs=Sfs_t, bins=bins, bp=bp, gamma=gamma)
+    Ssq = _apply_squeezing(Sx, squeezing).contiguous()
+    Tx = torch.empty_like(Sx)
+    cst = torch.full((rows,), const, dtype=torch.float32, device=dev)
+    _lib.squeeze(Ssq, bins, cst, rows, Tx)
+    if flipud:
+        ssq_freqs = ssq_freqs[::-1]
+    if not get_dWx:
+        dSx = None
+    Tx, Sx, w, dSx = [to_output(v, astensor) for v in (Tx, Sx, w, dSx)]
+    if get_w and get_dWx:
+        return Tx, Sx, ssq_freqs, Sfs, w, dSx
+    elif get_w:
+        return Tx, Sx, ssq_freqs, Sfs, w
+    elif get_dWx:
+        return Tx, Sx, ssq_freqs, Sfs, dSx
+    return Tx, Sx, ssq_freqs, Sfs
+
+
+def phase_stft(Sx, dSx, Sfs, gamma=None, parallel=None):
+    """w = |Sfs[k] - Im(dSx/Sx)/2pi|, inf where |Sx| < gamma
+    (_ssq_stft.py:201-246); default gamma = 10 eps32."""
+    import torch
+    Sx = as_device_tensor(Sx, complex_=True)
+    dSx = as_device_tensor(dSx, like=Sx, complex_=True)
+    Sfs = as_device_tensor(np.asarray(Sfs.detach().cpu() if hasattr(Sfs, 'detach')
+                                      else Sfs, dtype=np.float32), like=Sx, real=True)
+    if gamma is None:
+        gamma = 10 * EPS32
+    w = torch.empty(Sx.shape, dtype=torch.float32, device=Sx.device)
+    _lib.phase(Sx, dSx, Sfs, gamma, w)
+    return w

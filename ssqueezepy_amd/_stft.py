@@ -1,0 +1,119 @@
+"""Short-time Fourier transform on the GPU (stft of _stft.py:13-181).
+
+Windows are designed on the host (they are n_fft-point vectors) exactly as
+get_window does (_stft.py:259-310); framing, windowing and the per-frame FFTs
+of Sx and dSx run in ``ssqk_stft_exec``.
+"""
+import numpy as np
+import scipy.signal as sig
+
+from . import _lib
+from ._common import WARN, _process_fs_and_t
+from ._tensors import signal_to_device, to_output
+from .configs import gdefaults
+from .wavelets import _xifn
+
+__all__ = ['stft', 'get_window']
+
+
+def _zero_denormals(x):
+    """Zero |x| < 1000 * tiny (algos.py:593-613)."""
+    tiny = 1000 * np.finfo(x.dtype).tiny
+    x[(x < tiny) & (x > -tiny)] = 0
+    return x
+
+
+def get_window(window, win_len, n_fft=None, derivative=False, dtype=None):
+    """Window of length n_fft (padded if win_len < n_fft) and, with
+    ``derivative``, its frequency-domain derivative Re ifft(fft(w) i xi) with
+    the Nyquist xi zeroed (_stft.py:259-310)."""
+    if n_fft is None:
+        pl, pr = 0, 0
+    else:
+        if win_len > n_fft:
+            raise ValueError("Can't have `win_len > n_fft` ({} > {})".format(
+                win_len, n_fft))
+        pl = (n_fft - win_len) // 2
+        pr = (n_fft - win_len - pl)
+    if window is not None:
+        if isinstance(window, str):
+            window = sig.get_window(window, win_len, fftbins=True)
+        elif isinstance(window, np.ndarray):
+            if len(window) != win_len:
+                WARN("len(window) != win_len (%s != %s)" % (len(window), win_len))
+        else:
+            raise ValueError("`window` must be string or np.ndarray "
+                             "(got %s)" % window)
+    else:
+        window = sig.windows.dpss(win_len, max(4, win_len // 8), sym=False)
+    if len(window) < (win_len + pl + pr):
+        window = np.pad(window, [pl, pr])
+    if derivative:
+        wf = np.fft.fft(window)
+        Nw = len(window)
+        xi = _xifn(1, Nw)
+        if Nw % 2 == 0:
+            xi[Nw // 2] = 0
+        diff_window = np.fft.ifft(wf * 1j * xi).real
+    dtype = gdefaults('_stft.stft', dtype=dtype)['dtype']
+    window = _zero_denormals(np.asarray(window).astype(dtype))
+    if derivative:
+        diff_window = _zero_denormals(np.asarray(diff_window).astype(dtype))
+        return window, diff_window
+    return window
+
+
+def _check_NOLA(window, hop_len, dtype=None):
+    """Invertibility warnings of _stft.py:313-335 (host, scipy)."""
+    if hop_len > len(window):
+        WARN("`hop_len > len(window)`; STFT not invertible")
+    elif not sig.check_NOLA(window, len(window), len(window) - hop_len):
+        WARN("`window` fails Non-zero Overlap Add (NOLA) criterion; "
+             "STFT not invertible")
+    if dtype is None:
+        dtype = str(window.dtype)
+    if dtype == 'float32' and not sig.check_NOLA(
+            window, len(window), len(window) - hop_len, tol=1e-3):
+        WARN("Imprecision expected at right-most hop of signal, in inversion. "
+             "Lower `hop_len`, choose wider `window`, or use `dtype='float64'`.")
+
+
+def stft_setup(x, window, n_fft, win_len, hop_len, fs, t, padtype, dtype):
+    N = x.shape[-1]
+    _, fs, _ = _process_fs_and_t(fs, t, N)
+    n_fft = n_fft or min(N // hop_len, 512)
+    if win_len is None:
+        win_len = len(window) if isinstance(window, np.ndarray) else n_fft
+    dtype = gdefaults('_stft.stft', dtype=dtype)['dtype']
+    window, diff_window = get_window(window, win_len, n_fft, derivative=True,
+                                     dtype=dtype)
+    _check_NOLA(window, hop_len, dtype)
+    if padtype not in ('reflect', 'zero'):
+        raise NotImplementedError("stft padtype must be 'reflect' or 'zero' on "
+                                  "the GPU path (got %s)" % padtype)
+    if n_fft & (n_fft - 1):
+        raise NotImplementedError("stft n_fft must be a power of two on the GPU "
+                                  "path (got %d)" % n_fft)
+    return N, fs, n_fft, window, diff_window, dtype
+
+
+def stft(x, window=None, n_fft=None, win_len=None, hop_len=1, fs=None, t=None,
+         padtype='reflect', modulated=True, derivative=False, dtype=None):
+    """STFT of 1D/2D ``x``: Sx [n_fft//2+1, n_frames] (batched: leading dim),
+    and dSx with ``derivative`` (_stft.py:13-181).  Tensors on the GPU."""
+    import torch
+    assert x.ndim in (1, 2)
+    N, fs, n_fft, window, diff_window, dtype = stft_setup(
+        x, window, n_fft, win_len, hop_len, fs, t, padtype, dtype)
+    x_dev = signal_to_device(x, dtype)
+    dev = x_dev.device
+    n_frames = (N - 1) // hop_len + 1
+    rows = n_fft // 2 + 1
+    shape = (rows, n_frames) if x.ndim == 1 else (x.shape[0], rows, n_frames)
+    Sx = torch.empty(shape, dtype=torch.complex64, device=dev)
+    dSx = torch.empty_like(Sx) if derivative else None
+    win = torch.as_tensor(window, device=dev)
+    dwin = torch.as_tensor(diff_window, device=dev) if derivative else None
+    _lib.stft_exec(x_dev, n_fft, hop_len, padtype, modulated, win, dwin, fs, Sx,
+                   dSx)
+    return (Sx, dSx) if derivative else Sx

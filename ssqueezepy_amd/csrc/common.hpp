@@ -1,0 +1,158 @@
+// Shared host/device definitions of the ssqk library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/ssqk.h"
+
+namespace ssqk {
+
+constexpr int kNoBin = 0xFFFF;          // bin sentinel: point below gamma
+constexpr float kTwoPiF = 6.283185307179586f;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// Wavelet parameters in float64.  GMW (kind 0): a=gamma, b=beta, c=ln wc,
+// d=wc^gamma.  Morlet (kind 1): a=mu, b=ks, c=C0 (-1/2), d=C1.
+struct WavDev {
+  int kind;
+  double a, b, c, d;
+};
+
+// psih(w) in float64 (_gmw.py:203-210, wavelets.py:521-523).
+__device__ __forceinline__ double psih_eval(const WavDev& p, double w) {
+  if (p.kind == SSQK_WAVELET_GMW) {
+    if (!(w > 0.0)) return 0.0;
+    const double lw = log(w);
+    return 2.0 * exp(p.b * (lw - p.c) - (exp(p.a * lw) - p.d));
+  }
+  const double u = w - p.a;
+  return p.d * (exp(p.c * u * u) - p.b * exp(p.c * w * w));
+}
+
+// Squeeze bin parameters (algos.py:44-123, 356-374).  type: 0 linear,
+// 1 log, 2 log-piecewise.  For log types v0/v1 are log2 of the grid's
+// first (and transition) frequency and d0/d1 the log2 steps.
+struct BinDev {
+  int type, flip, omax, idx1;
+  double v0, d0, v1, d1;
+  float v0f, inv_d0f, v1f, inv_d1f;
+};
+
+// Exact-arithmetic bin of a float64 phase-transform value, following the
+// reference's clamp and round-half-even order (algos.py:859-953).
+__device__ __forceinline__ int bin_exact(double w, const BinDev& p) {
+  double k;
+  if (p.type == 0) {
+    k = fmin(rint(fmax((w - p.v0) / p.d0, 0.0)), (double)p.omax);
+  } else if (p.type == 1) {
+    k = fmin(rint(fmax((log2(w) - p.v0) / p.d0, 0.0)), (double)p.omax);
+  } else {
+    const double wl = log2(w);
+    if (wl > p.v1) k = fmin(rint((wl - p.v1) / p.d1) + p.idx1, (double)p.omax);
+    else k = fmax(rint((wl - p.v0) / p.d0), 0.0);
+  }
+  if (!(k >= 0.0)) k = 0.0;   // NaN guard
+  int ki = (int)k;
+  return p.flip ? p.omax - ki : ki;
+}
+
+// Bin of a phase-transform value given as the float32 inputs A+iB (dWx) and
+// C+iD (Wx).  The fast float32 evaluation is used unless the value lies
+// within 1e-3 of a rounding edge (or of the piecewise transition), where the
+// float64 evaluation of the same inputs decides.
+__device__ __forceinline__ int bin_from_dwx(float A, float B, float C, float D,
+                                            const BinDev& p) {
+  const float m2 = C * C + D * D;
+  const float w = fabsf((B * C - A * D) / (m2 * kTwoPiF));
+  float u;
+  bool near = false;
+  if (p.type == 0) {
+    u = (w - p.v0f) * p.inv_d0f;
+  } else {
+    const float wl = __log2f(w);
+    if (p.type == 2) {
+      const float dd = wl - p.v1f;
+      near = fabsf(dd) < 1e-4f * (1.0f + fabsf(p.v1f));
+      u = dd > 0.0f ? dd * p.inv_d1f : (wl - p.v0f) * p.inv_d0f;
+    } else {
+      u = (wl - p.v0f) * p.inv_d0f;
+    }
+  }
+  const float fr = u - floorf(u);
+  near = near || fabsf(fr - 0.5f) < 1e-3f || !(fabsf(u) < 1e6f);
+  if (near) {
+    const double wd = fabs(((double)B * C - (double)A * D) /
+                           (((double)C * C + (double)D * D) * kTwoPi));
+    return bin_exact(wd, p);
+  }
+  float k;
+  if (p.type == 0 || p.type == 1) {
+    k = fminf(rintf(fmaxf(u, 0.0f)), (float)p.omax);
+  } else {
+    const float wl = __log2f(w);
+    if (wl > p.v1f) k = fminf(rintf(u) + (float)p.idx1, (float)p.omax);
+    else k = fmaxf(rintf(u), 0.0f);
+  }
+  const int ki = (int)k;
+  return p.flip ? p.omax - ki : ki;
+}
+
+// Same for an already computed float32 w (indexed_sum_onfly path,
+// algos.py:153-250): caller has checked that w is finite.
+__device__ __forceinline__ int bin_from_w(float w, const BinDev& p) {
+  float u;
+  bool near = false;
+  if (p.type == 0) {
+    u = (w - p.v0f) * p.inv_d0f;
+  } else {
+    const float wl = __log2f(w);
+    if (p.type == 2) {
+      const float dd = wl - p.v1f;
+      near = fabsf(dd) < 1e-4f * (1.0f + fabsf(p.v1f));
+      u = dd > 0.0f ? dd * p.inv_d1f : (wl - p.v0f) * p.inv_d0f;
+    } else {
+      u = (wl - p.v0f) * p.inv_d0f;
+    }
+  }
+  const float fr = u - floorf(u);
+  near = near || fabsf(fr - 0.5f) < 1e-3f || !(fabsf(u) < 1e6f);
+  if (near) return bin_exact((double)w, p);
+  float k;
+  if (p.type == 0 || p.type == 1) {
+    k = fminf(rintf(fmaxf(u, 0.0f)), (float)p.omax);
+  } else {
+    const float wl = __log2f(w);
+    if (wl > p.v1f) k = fminf(rintf(u) + (float)p.idx1, (float)p.omax);
+    else k = fmaxf(rintf(u), 0.0f);
+  }
+  const int ki = (int)k;
+  return p.flip ? p.omax - ki : ki;
+}
+
+inline BinDev make_bin_dev(const ssqk_bin_params* bp) {
+  BinDev d{};
+  d.type = bp->type;
+  d.flip = bp->flipud;
+  d.omax = (int)bp->omax;
+  d.idx1 = (int)bp->idx1;
+  d.v0 = bp->v0; d.d0 = bp->d0; d.v1 = bp->v1; d.d1 = bp->d1;
+  d.v0f = (float)bp->v0; d.inv_d0f = (float)(1.0 / bp->d0);
+  d.v1f = (float)bp->v1; d.inv_d1f = (float)(bp->type == 2 ? 1.0 / bp->d1 : 0.0);
+  return d;
+}
+
+}  // namespace ssqk
+
+// ------------------------------------------------------------- host side
+namespace ssqk_host {
+void set_error(const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+// Twiddle table exp(+2 pi i t / M), t in [0, M), owned by the library for
+// the process lifetime, one per (device, M).
+const float2* twiddles(int M);
+}  // namespace ssqk_host
+
+#define SSQK_CHECK(expr)                                                  \
+  do {                                                                    \
+    hipError_t _e = (expr);                                               \
+    if (_e != hipSuccess) return ssqk_host::check_hip(_e, #expr);         \
+  } while (0)

@@ -1,0 +1,722 @@
+// Continuous wavelet transform on gfx950: padded forward FFT, band-limited
+// inverse transforms per scale, and the fused phase-transform / squeeze-bin
+// output stage.
+//
+// Layout in HBM (per signal b of a batch):
+//   x    [N] float32                      input
+//   xh   [L/2+1] complex64  (workspace)   FFT of the padded signal
+//   Wx   [na, N] complex64                CWT, unpadded (written once)
+//   bins [na, N] uint16                   squeeze row per point
+//
+// Scale i needs y = ifft_L(psih_i * xh) on n in [n1, n1+N).  psih_i is
+// non-negligible only on a band [ka, kb] of positive bins.  With M >= kb-ka+1
+// a power of two and D = L/M:
+//   y[a + D q] = (1/L) w_L^{ka (a + D q)} * sum_{m<M} (X[ka+m] w_L^{m a}) w_M^{m q}
+// so each phase a in [0, D) is one M-point FFT ("band" path, M <= 8192, one
+// workgroup per group of phases, everything in LDS).  Wider bands use the
+// full-length four-step FFT L = L1*L2 through a workspace ("full" path).
+#include "common.hpp"
+#include "fft_core.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace ssqk;
+
+namespace {
+
+constexpr int kP = 8192;    // complex points per workgroup and channel
+constexpr int kNT = 512;    // threads per FFT workgroup
+constexpr int kSmallFwd = 16384;  // largest L done by one workgroup
+
+// phases per workgroup of the band path: kP / M, capped so the row padding of
+// tiny transforms stays within LDS
+constexpr int band_rows(int M) { return kP / M > 1024 ? 1024 : kP / M; }
+
+struct CwtArgs {
+  const float2* xh;
+  int64_t xh_stride;        // complex elements between signals in xh
+  float2* Wx;
+  float2* dWx;
+  float* w;
+  uint16_t* bins;
+  int64_t out_stride;       // na * N
+  const double* scales;
+  int L, N, n1;
+  int strict;
+  double inv_dt;
+  float gamma2;
+  WavDev wav;
+  BinDev bp;
+};
+
+// Source index of padded sample n (numpy pad modes, utils/common.py:123-158);
+// -1 for a zero sample.
+__device__ __forceinline__ int64_t pad_src(int64_t n, int64_t N, int64_t n1, int type) {
+  int64_t t = n - n1;
+  if (t >= 0 && t < N) return t;
+  switch (type) {
+    case SSQK_PAD_REFLECT: {
+      if (N == 1) return 0;
+      const int64_t P = 2 * (N - 1);
+      int64_t m = t % P;
+      if (m < 0) m += P;
+      return m < N ? m : P - m;
+    }
+    case SSQK_PAD_SYMMETRIC: {
+      const int64_t P = 2 * N;
+      int64_t m = t % P;
+      if (m < 0) m += P;
+      return m < N ? m : P - 1 - m;
+    }
+    case SSQK_PAD_REPLICATE:
+      return t < 0 ? 0 : N - 1;
+    case SSQK_PAD_WRAP: {
+      int64_t m = t % N;
+      return m < 0 ? m + N : m;
+    }
+    default:
+      return -1;
+  }
+}
+
+// Shared output stage: z, zd are Wx and dWx at padded position n.
+__device__ __forceinline__ void emit(const CwtArgs& a, int b, int i, int64_t n,
+                                     float2 z, float2 zd) {
+  const int64_t j = n - a.n1;
+  if (j < 0 || j >= a.N) return;
+  const int64_t o = (int64_t)b * a.out_stride + (int64_t)i * a.N + j;
+  a.Wx[o] = z;
+  if (a.dWx) a.dWx[o] = zd;
+  const float m2 = z.x * z.x + z.y * z.y;
+  float wv = 0.f;
+  if (a.w || (a.bins && !a.strict)) {
+    wv = (m2 < a.gamma2) ? INFINITY
+                         : fabsf((zd.y * z.x - zd.x * z.y) / (m2 * kTwoPiF));
+    if (a.w) a.w[o] = wv;
+  }
+  if (a.bins) {
+    uint16_t k = (uint16_t)kNoBin;
+    if (a.strict) {
+      if (m2 > a.gamma2) k = (uint16_t)bin_from_dwx(zd.x, zd.y, z.x, z.y, a.bp);
+    } else if (!isinf(wv)) {
+      k = (uint16_t)bin_from_w(wv, a.bp);
+    }
+    a.bins[o] = k;
+  }
+}
+
+// X = psih(s xi_k) xh[k] and Xd = X * i xi_k / dt for a positive bin k.
+__device__ __forceinline__ void band_value(const CwtArgs& a, const float2* xh,
+                                           double s, int k, float2& X, float2& Xd) {
+  const int half = a.L >> 1;
+  const double xi = (double)k * (kTwoPi / (double)a.L);
+  double ps = psih_eval(a.wav, s * xi);
+  if (k == half) ps *= 0.5;                 // Nyquist halving (wavelets.py:86-95)
+  const float2 v = xh[k];
+  const float pf = (float)ps;
+  X = make_float2(pf * v.x, pf * v.y);
+  const float q = (float)(xi * a.inv_dt);
+  Xd = make_float2(-q * X.y, q * X.x);
+}
+
+// ------------------------------------------------------------- band path
+// task = {scale i, first phase a0, band start ka, log2 D}
+template <int M>
+__global__ __launch_bounds__(kNT) void k_cwt_band(CwtArgs a, const int4* __restrict__ tasks,
+                                                  const float2* __restrict__ tw) {
+  constexpr int ROWS = band_rows(M);
+  __shared__ float2 sW[Lds<M>::elems(ROWS)];
+  __shared__ float2 sD[Lds<M>::elems(ROWS)];
+  const int4 t = tasks[blockIdx.x];
+  const int b = blockIdx.y;
+  const int i = t.x, a0 = t.y, ka = t.z, D = 1 << t.w;
+  const int L = a.L, half = L >> 1;
+  const float2* xh = a.xh + (int64_t)b * a.xh_stride;
+  const double s = a.scales[i];
+
+  // band samples into row 0
+  for (int m = threadIdx.x; m < M; m += kNT) {
+    float2 X = make_float2(0.f, 0.f), Xd = X;
+    const int k = ka + m;
+    if (k <= half) band_value(a, xh, s, k, X, Xd);
+    sW[Lds<M>::idx(0, m)] = X;
+    sD[Lds<M>::idx(0, m)] = Xd;
+  }
+  __syncthreads();
+  // phase ramps w_L^{m (a0 + r)}: rows r >= 1 first, then row 0 in place
+  for (int e = threadIdx.x + M; e < ROWS * M; e += kNT) {
+    const int r = e / M, m = e - r * M;
+    const float2 ph = unit_phase((uint64_t)m * (uint64_t)(a0 + r), L);
+    sW[Lds<M>::idx(r, m)] = cmul(sW[Lds<M>::idx(0, m)], ph);
+    sD[Lds<M>::idx(r, m)] = cmul(sD[Lds<M>::idx(0, m)], ph);
+  }
+  __syncthreads();
+  if (a0 != 0) {
+    for (int m = threadIdx.x; m < M; m += kNT) {
+      const float2 ph = unit_phase((uint64_t)m * (uint64_t)a0, L);
+      sW[Lds<M>::idx(0, m)] = cmul(sW[Lds<M>::idx(0, m)], ph);
+      sD[Lds<M>::idx(0, m)] = cmul(sD[Lds<M>::idx(0, m)], ph);
+    }
+  }
+  float2* const bufs[2] = {sW, sD};
+  lds_fft<M, ROWS, +1, kNT, 2>(bufs, tw);
+
+  const float invL = 1.0f / (float)L;
+  for (int e = threadIdx.x; e < ROWS * M; e += kNT) {
+    const int r = e % ROWS, q = e / ROWS;
+    const int ph_a = a0 + r;
+    if (ph_a >= D) continue;
+    const int64_t n = ph_a + (int64_t)D * q;
+    if (n < a.n1 || n >= (int64_t)a.n1 + a.N) continue;
+    float2 ph = unit_phase((uint64_t)ka * (uint64_t)n, L);
+    ph.x *= invL;
+    ph.y *= invL;
+    emit(a, b, i, n, cmul(sW[Lds<M>::idx(r, q)], ph), cmul(sD[Lds<M>::idx(r, q)], ph));
+  }
+}
+
+// ------------------------------------------------------------- full path
+// L = L1 * L2, k = k1 + L1 k2, n = L2 n1 + n2:
+//   y[L2 n1 + n2] = sum_k1 w_L1^{k1 n1} [ w_L^{k1 n2} sum_k2 X[k1 + L1 k2] w_L2^{k2 n2} ]
+// step A (rows = consecutive k1): FFT over k2, twiddle, store S[n2][k1]
+// step B (rows = consecutive n2): FFT over k1, output stage.
+// taskA = {i, k1_0, slot, -}, taskB = {i, n2_0, slot, -}; band[i] = {ka, kb}.
+template <int L2>
+__global__ __launch_bounds__(kNT) void k_cwt_fullA(CwtArgs a, const int4* __restrict__ tasks,
+                                                   const int2* __restrict__ band,
+                                                   const float2* __restrict__ tw, int L1,
+                                                   int batch, float2* __restrict__ scrW,
+                                                   float2* __restrict__ scrD) {
+  constexpr int G = kP / L2;
+  __shared__ float2 sW[Lds<L2>::elems(G)];
+  __shared__ float2 sD[Lds<L2>::elems(G)];
+  const int4 t = tasks[blockIdx.x];
+  const int b = blockIdx.y;
+  const int i = t.x, k1_0 = t.y, slot = t.z;
+  const int2 kab = band[i];
+  const float2* xh = a.xh + (int64_t)b * a.xh_stride;
+  const double s = a.scales[i];
+  for (int e = threadIdx.x; e < G * L2; e += kNT) {
+    const int g = e % G, k2 = e / G;
+    const int k = k1_0 + g + L1 * k2;
+    float2 X = make_float2(0.f, 0.f), Xd = X;
+    if (k >= kab.x && k <= kab.y) band_value(a, xh, s, k, X, Xd);
+    sW[Lds<L2>::idx(g, k2)] = X;
+    sD[Lds<L2>::idx(g, k2)] = Xd;
+  }
+  float2* const bufs[2] = {sW, sD};
+  lds_fft<L2, G, +1, kNT, 2>(bufs, tw);
+  const int64_t base = ((int64_t)slot * batch + b) * (int64_t)L2;
+  for (int e = threadIdx.x; e < G * L2; e += kNT) {
+    const int g = e % G, n2 = e / G;
+    const int k1 = k1_0 + g;
+    const float2 ph = unit_phase((uint64_t)k1 * (uint64_t)n2, a.L);
+    const int64_t o = (base + n2) * L1 + k1;
+    scrW[o] = cmul(sW[Lds<L2>::idx(g, n2)], ph);
+    scrD[o] = cmul(sD[Lds<L2>::idx(g, n2)], ph);
+  }
+}
+
+template <int L1>
+__global__ __launch_bounds__(kNT) void k_cwt_fullB(CwtArgs a, const int4* __restrict__ tasks,
+                                                   const float2* __restrict__ tw, int L2,
+                                                   int batch, const float2* __restrict__ scrW,
+                                                   const float2* __restrict__ scrD) {
+  constexpr int G = kP / L1;
+  __shared__ float2 sW[Lds<L1>::elems(G)];
+  __shared__ float2 sD[Lds<L1>::elems(G)];
+  const int4 t = tasks[blockIdx.x];
+  const int b = blockIdx.y;
+  const int i = t.x, n2_0 = t.y, slot = t.z;
+  const int64_t base = ((int64_t)slot * batch + b) * (int64_t)L2;
+  for (int e = threadIdx.x; e < G * L1; e += kNT) {
+    const int k1 = e % L1, g = e / L1;
+    const int64_t o = (base + n2_0 + g) * L1 + k1;
+    sW[Lds<L1>::idx(g, k1)] = scrW[o];
+    sD[Lds<L1>::idx(g, k1)] = scrD[o];
+  }
+  float2* const bufs[2] = {sW, sD};
+  lds_fft<L1, G, +1, kNT, 2>(bufs, tw);
+  const float invL = 1.0f / (float)a.L;
+  for (int e = threadIdx.x; e < G * L1; e += kNT) {
+    const int g = e % G, n1 = e / G;
+    const int64_t n = (int64_t)L2 * n1 + n2_0 + g;
+    if (n < a.n1 || n >= (int64_t)a.n1 + a.N) continue;
+    float2 z = sW[Lds<L1>::idx(g, n1)], zd = sD[Lds<L1>::idx(g, n1)];
+    z.x *= invL; z.y *= invL; zd.x *= invL; zd.y *= invL;
+    emit(a, b, i, n, z, zd);
+  }
+}
+
+// --------------------------------------------------- forward FFT of x_pad
+struct FwdArgs {
+  const float* x;
+  int64_t N, n1;
+  int L, pad;
+  float2* xh;
+  int64_t xh_stride;
+};
+
+template <int L>
+__global__ __launch_bounds__(kNT) void k_fwd_small(FwdArgs f, const float2* __restrict__ tw) {
+  __shared__ float2 s[Lds<L>::elems(1)];
+  const int b = blockIdx.x;
+  const float* x = f.x + (int64_t)b * f.N;
+  for (int n = threadIdx.x; n < L; n += kNT) {
+    const int64_t src = pad_src(n, f.N, f.n1, f.pad);
+    s[Lds<L>::idx(0, n)] = make_float2(src >= 0 ? x[src] : 0.f, 0.f);
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L, 1, -1, kNT, 1>(bufs, tw);
+  float2* xh = f.xh + (int64_t)b * f.xh_stride;
+  for (int k = threadIdx.x; k <= L / 2; k += kNT) xh[k] = s[Lds<L>::idx(0, k)];
+}
+
+template <int L2>
+__global__ __launch_bounds__(kNT) void k_fwd_A(FwdArgs f, const float2* __restrict__ tw, int L1,
+                                               float2* __restrict__ scr) {
+  constexpr int G = kP / L2;
+  __shared__ float2 s[Lds<L2>::elems(G)];
+  const int b = blockIdx.y;
+  const int k1_0 = blockIdx.x * G;
+  const float* x = f.x + (int64_t)b * f.N;
+  for (int e = threadIdx.x; e < G * L2; e += kNT) {
+    const int g = e % G, k2 = e / G;
+    const int64_t src = pad_src(k1_0 + g + (int64_t)L1 * k2, f.N, f.n1, f.pad);
+    s[Lds<L2>::idx(g, k2)] = make_float2(src >= 0 ? x[src] : 0.f, 0.f);
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L2, G, -1, kNT, 1>(bufs, tw);
+  for (int e = threadIdx.x; e < G * L2; e += kNT) {
+    const int g = e % G, n2 = e / G;
+    const int k1 = k1_0 + g;
+    const float2 ph = cconj(unit_phase((uint64_t)k1 * (uint64_t)n2, f.L));
+    scr[((int64_t)b * L2 + n2) * L1 + k1] = cmul(s[Lds<L2>::idx(g, n2)], ph);
+  }
+}
+
+template <int L1>
+__global__ __launch_bounds__(kNT) void k_fwd_B(FwdArgs f, const float2* __restrict__ tw, int L2,
+                                               const float2* __restrict__ scr) {
+  constexpr int G = kP / L1;
+  __shared__ float2 s[Lds<L1>::elems(G)];
+  const int b = blockIdx.y;
+  const int n2_0 = blockIdx.x * G;
+  for (int e = threadIdx.x; e < G * L1; e += kNT) {
+    const int k1 = e % L1, g = e / L1;
+    s[Lds<L1>::idx(g, k1)] = scr[((int64_t)b * L2 + n2_0 + g) * L1 + k1];
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L1, G, -1, kNT, 1>(bufs, tw);
+  float2* xh = f.xh + (int64_t)b * f.xh_stride;
+  for (int e = threadIdx.x; e < G * L1; e += kNT) {
+    const int g = e % G, n1 = e / G;
+    const int64_t k = (int64_t)L2 * n1 + n2_0 + g;
+    if (k <= f.L / 2) xh[k] = s[Lds<L1>::idx(g, n1)];
+  }
+}
+
+// ------------------------------------------------------------ dispatch
+#define SSQK_POW2_CASES(MACRO) \
+  MACRO(2) MACRO(4) MACRO(8) MACRO(16) MACRO(32) MACRO(64) MACRO(128) MACRO(256) \
+  MACRO(512) MACRO(1024) MACRO(2048) MACRO(4096) MACRO(8192)
+
+int launch_band(int M, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks) {
+  switch (M) {
+#define C(m)                                                                       \
+  case m:                                                                          \
+    hipLaunchKernelGGL(k_cwt_band<m>, grid, dim3(kNT), 0, st, a, tasks,            \
+                       ssqk_host::twiddles(m));                                    \
+    break;
+    SSQK_POW2_CASES(C)
+#undef C
+    default:
+      ssqk_host::set_error("band size %d unsupported", M);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  return SSQK_OK;
+}
+
+int launch_fullA(int L2, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks,
+                 const int2* band, int L1, int batch, float2* sW, float2* sD) {
+  switch (L2) {
+#define C(m)                                                                       \
+  case m:                                                                          \
+    hipLaunchKernelGGL(k_cwt_fullA<m>, grid, dim3(kNT), 0, st, a, tasks, band,     \
+                       ssqk_host::twiddles(m), L1, batch, sW, sD);                 \
+    break;
+    C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+    default:
+      ssqk_host::set_error("four-step size %d unsupported", L2);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  return SSQK_OK;
+}
+
+int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks,
+                 int L2, int batch, const float2* sW, const float2* sD) {
+  switch (L1) {
+#define C(m)                                                                       \
+  case m:                                                                          \
+    hipLaunchKernelGGL(k_cwt_fullB<m>, grid, dim3(kNT), 0, st, a, tasks,           \
+                       ssqk_host::twiddles(m), L2, batch, sW, sD);                 \
+    break;
+    C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+    default:
+      ssqk_host::set_error("four-step size %d unsupported", L1);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  return SSQK_OK;
+}
+
+int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, float2* scr) {
+  const int L = f.L;
+  if (L <= kSmallFwd) {
+    switch (L) {
+#define C(m)                                                                       \
+  case m:                                                                          \
+    hipLaunchKernelGGL(k_fwd_small<m>, dim3(batch), dim3(kNT), 0, st, f,           \
+                       ssqk_host::twiddles(m));                                    \
+    break;
+      SSQK_POW2_CASES(C) C(16384)
+#undef C
+      default:
+        ssqk_host::set_error("padded length %d unsupported", L);
+        return SSQK_ERR_UNSUPPORTED;
+    }
+    return SSQK_OK;
+  }
+  switch (L2) {
+#define C(m)                                                                       \
+  case m:                                                                          \
+    hipLaunchKernelGGL(k_fwd_A<m>, dim3(L1 / (kP / m), batch), dim3(kNT), 0, st, f,  \
+                       ssqk_host::twiddles(m), L1, scr);                           \
+    break;
+    C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+    default:
+      ssqk_host::set_error("four-step size %d unsupported", L2);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  switch (L1) {
+#define C(m)                                                                       \
+  case m:                                                                          \
+    hipLaunchKernelGGL(k_fwd_B<m>, dim3(L2 / (kP / m), batch), dim3(kNT), 0, st, f,  \
+                       ssqk_host::twiddles(m), L2, scr);                           \
+    break;
+    C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+    default:
+      ssqk_host::set_error("four-step size %d unsupported", L1);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  return SSQK_OK;
+}
+
+// ----------------------------------------------------------- host math
+double psih_host(const WavDev& p, double w) {
+  if (p.kind == SSQK_WAVELET_GMW) {
+    if (!(w > 0.0)) return 0.0;
+    const double lw = std::log(w);
+    return 2.0 * std::exp(p.b * (lw - p.c) - (std::exp(p.a * lw) - p.d));
+  }
+  const double u = w - p.a;
+  return p.d * (std::exp(p.c * u * u) - p.b * std::exp(p.c * w * w));
+}
+
+// [w_lo, w_hi]: |psih| >= rel * |psih(center)| inside (unimodal wavelets)
+void band_limits(const WavDev& p, double center, double rel, double* lo, double* hi) {
+  const double th = rel * std::fabs(psih_host(p, center));
+  double a = 1e-12, b = center;
+  if (std::fabs(psih_host(p, a)) >= th) {
+    *lo = 0.0;
+  } else {
+    for (int it = 0; it < 200; ++it) {
+      const double m = 0.5 * (a + b);
+      if (std::fabs(psih_host(p, m)) >= th) b = m; else a = m;
+    }
+    *lo = a * (1 - 1e-9);
+  }
+  double c = center, d = 2 * center + 10;
+  while (std::fabs(psih_host(p, d)) >= th && d < 1e300) d *= 2;
+  for (int it = 0; it < 200; ++it) {
+    const double m = 0.5 * (c + d);
+    if (std::fabs(psih_host(p, m)) >= th) c = m; else d = m;
+  }
+  *hi = d * (1 + 1e-9);
+}
+
+int ilog2(int64_t v) {
+  int r = 0;
+  while ((int64_t(1) << r) < v) ++r;
+  return r;
+}
+
+}  // namespace
+
+// ================================================================== plan
+struct ssqk_cwt_plan_s {
+  int64_t N = 0, L = 0, n1 = 0;
+  int pad = 0;
+  int64_t na = 0;
+  double dt = 1.0;
+  WavDev wav{};
+  double* d_scales = nullptr;
+  int2* d_band = nullptr;
+  struct Cls {
+    int M = 0;
+    int ntasks = 0;
+    int4* d_tasks = nullptr;
+  };
+  std::vector<Cls> classes;
+  // full path
+  int L1 = 0, L2 = 0;
+  std::vector<int> full;      // scale indices
+  int slots = 0;              // scales per chunk
+  struct Chunk {
+    int nA = 0, nB = 0;
+    int4* d_A = nullptr;
+    int4* d_B = nullptr;
+  };
+  std::vector<Chunk> chunks;
+  int fL1 = 0, fL2 = 0;       // forward four-step split (L > kSmallFwd)
+};
+
+static void split_four_step(int64_t L, int* L1, int* L2) {
+  const int lg = ilog2(L);
+  *L2 = 1 << ((lg + 1) / 2);
+  *L1 = (int)(L / *L2);
+}
+
+extern "C" int ssqk_cwt_plan_create(ssqk_cwt_plan* out, int64_t N, int64_t L, int64_t n1,
+                                    int pad_type, int wavelet_kind,
+                                    const double* wavelet_params, const double* scales,
+                                    int64_t na, double dt, double band_rel) {
+  if (!out || !wavelet_params || !scales) {
+    ssqk_host::set_error("null argument");
+    return SSQK_ERR_ARG;
+  }
+  *out = nullptr;
+  if (L < 2 || (L & (L - 1)) != 0 || L > (1 << 24)) {
+    ssqk_host::set_error("L=%lld must be a power of two in [2, 2^24]", (long long)L);
+    return SSQK_ERR_ARG;
+  }
+  if (N < 1 || n1 < 0 || n1 + N > L || na < 1 || na > 65535 || !(dt > 0)) {
+    ssqk_host::set_error("bad sizes N=%lld L=%lld n1=%lld na=%lld", (long long)N,
+                         (long long)L, (long long)n1, (long long)na);
+    return SSQK_ERR_ARG;
+  }
+  if (wavelet_kind != SSQK_WAVELET_GMW && wavelet_kind != SSQK_WAVELET_MORLET) {
+    ssqk_host::set_error("unknown wavelet kind %d", wavelet_kind);
+    return SSQK_ERR_UNSUPPORTED;
+  }
+  ssqk_cwt_plan_s* p = new (std::nothrow) ssqk_cwt_plan_s();
+  if (!p) {
+    ssqk_host::set_error("out of host memory");
+    return SSQK_ERR_ARG;
+  }
+  p->N = N; p->L = L; p->n1 = n1; p->pad = pad_type; p->na = na; p->dt = dt;
+  double center;
+  if (wavelet_kind == SSQK_WAVELET_GMW) {
+    const double gamma = wavelet_params[0], beta = wavelet_params[1];
+    if (!(gamma > 0) || !(beta > 0)) {
+      delete p;
+      ssqk_host::set_error("GMW needs gamma > 0 and beta > 0");
+      return SSQK_ERR_ARG;
+    }
+    const double wc = std::pow(beta / gamma, 1.0 / gamma);
+    p->wav = WavDev{SSQK_WAVELET_GMW, gamma, beta, std::log(wc), std::pow(wc, gamma)};
+    center = wc;
+  } else {
+    const double mu = wavelet_params[0];
+    const double cs = std::pow(1 + std::exp(-mu * mu) - 2 * std::exp(-0.75 * mu * mu), -0.5);
+    const double ks = std::exp(-0.5 * mu * mu);
+    p->wav = WavDev{SSQK_WAVELET_MORLET, mu, ks, -0.5,
+                    std::sqrt(2.0) * cs * std::pow(M_PI, 0.25)};
+    center = mu > 0 ? mu : 1.0;
+  }
+  double wlo, whi;
+  band_limits(p->wav, center, band_rel > 0 ? band_rel : 1e-10, &wlo, &whi);
+
+  const int64_t half = L / 2;
+  const double dxi = 2 * M_PI / (double)L;
+  std::vector<int2> band(na);
+  std::vector<std::vector<int4>> cls_tasks(14);
+  for (int64_t i = 0; i < na; ++i) {
+    const double s = scales[i];
+    if (!(s > 0)) {
+      delete p;
+      ssqk_host::set_error("scales must be positive (scale %lld = %g)", (long long)i, s);
+      return SSQK_ERR_ARG;
+    }
+    int64_t ka = (int64_t)std::ceil(wlo / (s * dxi));
+    int64_t kb = (int64_t)std::floor(whi / (s * dxi));
+    ka = std::max<int64_t>(ka, 0);
+    kb = std::min<int64_t>(kb, half);
+    if (ka > kb) { ka = 0; kb = -1; }        // empty band: row of zeros
+    band[i] = make_int2((int)ka, (int)kb);
+    const int64_t K = std::max<int64_t>(kb - ka + 1, 1);
+    int64_t M = std::max<int64_t>(int64_t(1) << ilog2(K), 2);
+    M = std::min<int64_t>(M, L);
+    if (M <= kP) {
+      const int lM = ilog2(M);
+      const int64_t D = L / M;
+      const int rows = band_rows((int)M);
+      for (int64_t a0 = 0; a0 < D; a0 += rows)
+        cls_tasks[lM].push_back(make_int4((int)i, (int)a0, (int)ka, ilog2(D)));
+    } else {
+      p->full.push_back((int)i);
+    }
+  }
+  hipError_t e = hipSuccess;
+  e = hipMalloc(&p->d_scales, sizeof(double) * na);
+  if (e == hipSuccess) e = hipMemcpy(p->d_scales, scales, sizeof(double) * na, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&p->d_band, sizeof(int2) * na);
+  if (e == hipSuccess) e = hipMemcpy(p->d_band, band.data(), sizeof(int2) * na, hipMemcpyHostToDevice);
+  for (int lM = 0; lM < 14 && e == hipSuccess; ++lM) {
+    if (cls_tasks[lM].empty()) continue;
+    ssqk_cwt_plan_s::Cls c;
+    c.M = 1 << lM;
+    c.ntasks = (int)cls_tasks[lM].size();
+    e = hipMalloc(&c.d_tasks, sizeof(int4) * c.ntasks);
+    if (e == hipSuccess)
+      e = hipMemcpy(c.d_tasks, cls_tasks[lM].data(), sizeof(int4) * c.ntasks, hipMemcpyHostToDevice);
+    p->classes.push_back(c);
+  }
+  if (!p->full.empty() && e == hipSuccess) {
+    split_four_step(L, &p->L1, &p->L2);
+    // scratch budget ~64 MiB per signal of the batch
+    const int64_t per_slot = L * 2 * (int64_t)sizeof(float2);
+    p->slots = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)p->full.size(),
+                                                            (64ll << 20) / per_slot));
+    const int GA = kP / p->L2, GB = kP / p->L1;
+    for (size_t c0 = 0; c0 < p->full.size() && e == hipSuccess; c0 += p->slots) {
+      std::vector<int4> A, B;
+      for (int sl = 0; sl < p->slots && c0 + sl < p->full.size(); ++sl) {
+        const int i = p->full[c0 + sl];
+        for (int k1 = 0; k1 < p->L1; k1 += GA) A.push_back(make_int4(i, k1, sl, 0));
+        for (int n2 = 0; n2 < p->L2; n2 += GB) B.push_back(make_int4(i, n2, sl, 0));
+      }
+      ssqk_cwt_plan_s::Chunk ch;
+      ch.nA = (int)A.size();
+      ch.nB = (int)B.size();
+      e = hipMalloc(&ch.d_A, sizeof(int4) * A.size());
+      if (e == hipSuccess) e = hipMemcpy(ch.d_A, A.data(), sizeof(int4) * A.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMalloc(&ch.d_B, sizeof(int4) * B.size());
+      if (e == hipSuccess) e = hipMemcpy(ch.d_B, B.data(), sizeof(int4) * B.size(), hipMemcpyHostToDevice);
+      p->chunks.push_back(ch);
+    }
+  }
+  if (L > kSmallFwd) split_four_step(L, &p->fL1, &p->fL2);
+  if (e != hipSuccess) {
+    ssqk_cwt_plan_destroy(p);
+    return ssqk_host::check_hip(e, "plan upload");
+  }
+  *out = p;
+  return SSQK_OK;
+}
+
+extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
+  if (!p) return SSQK_OK;
+  (void)hipFree(p->d_scales);
+  (void)hipFree(p->d_band);
+  for (auto& c : p->classes) (void)hipFree(c.d_tasks);
+  for (auto& c : p->chunks) {
+    (void)hipFree(c.d_A);
+    (void)hipFree(c.d_B);
+  }
+  delete p;
+  return SSQK_OK;
+}
+
+static int64_t xh_stride(const ssqk_cwt_plan_s* p) {
+  return ((p->L / 2 + 1 + 63) / 64) * 64;
+}
+
+static void ws_layout(const ssqk_cwt_plan_s* p, int64_t batch, size_t* off_fwd,
+                      size_t* off_full, size_t* total) {
+  size_t o = sizeof(float2) * (size_t)(xh_stride(p) * batch);
+  o = (o + 255) & ~size_t(255);
+  *off_fwd = o;
+  if (p->L > kSmallFwd) o += sizeof(float2) * (size_t)(p->L * batch);
+  o = (o + 255) & ~size_t(255);
+  *off_full = o;
+  if (!p->full.empty()) o += sizeof(float2) * (size_t)(2 * p->L * batch * p->slots);
+  *total = o;
+}
+
+extern "C" int ssqk_cwt_workspace_size(ssqk_cwt_plan p, int64_t batch, size_t* bytes) {
+  if (!p || !bytes || batch < 1) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_workspace_size");
+    return SSQK_ERR_ARG;
+  }
+  size_t a, b;
+  ws_layout(p, batch, &a, &b, bytes);
+  return SSQK_OK;
+}
+
+extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, void* Wx,
+                             void* dWx, float* w, uint16_t* bins, const ssqk_bin_params* bp,
+                             double gamma, int gamma_strict, void* workspace, void* stream) {
+  if (!p || !x || !Wx || !workspace || batch < 1 || batch > 65535) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_exec");
+    return SSQK_ERR_ARG;
+  }
+  if (bins && !bp) {
+    ssqk_host::set_error("bins requested without bin parameters");
+    return SSQK_ERR_ARG;
+  }
+  if (bins && bp && (bp->omax < 0 || bp->omax >= kNoBin)) {
+    ssqk_host::set_error("omax=%lld out of range for uint16 bins", (long long)bp->omax);
+    return SSQK_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  size_t off_fwd, off_full, total;
+  ws_layout(p, batch, &off_fwd, &off_full, &total);
+  char* ws = (char*)workspace;
+  float2* xh = (float2*)ws;
+
+  FwdArgs f;
+  f.x = x; f.N = p->N; f.n1 = p->n1; f.L = (int)p->L; f.pad = p->pad;
+  f.xh = xh; f.xh_stride = xh_stride(p);
+  int rc = launch_fwd(f, (int)batch, st, p->fL1, p->fL2, (float2*)(ws + off_fwd));
+  if (rc) return rc;
+  SSQK_CHECK(hipGetLastError());
+
+  CwtArgs a;
+  a.xh = xh; a.xh_stride = xh_stride(p);
+  a.Wx = (float2*)Wx; a.dWx = (float2*)dWx; a.w = w; a.bins = bins;
+  a.out_stride = p->na * p->N;
+  a.scales = p->d_scales;
+  a.L = (int)p->L; a.N = (int)p->N; a.n1 = (int)p->n1;
+  a.strict = gamma_strict;
+  a.inv_dt = 1.0 / p->dt;
+  a.gamma2 = (float)(gamma * gamma);
+  a.wav = p->wav;
+  ssqk_bin_params zero{};
+  a.bp = make_bin_dev(bp ? bp : &zero);
+  for (const auto& c : p->classes) {
+    rc = launch_band(c.M, dim3(c.ntasks, (unsigned)batch), st, a, c.d_tasks);
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+  }
+  float2* sW = (float2*)(ws + off_full);
+  float2* sD = sW + (size_t)p->L * batch * p->slots;
+  for (const auto& ch : p->chunks) {
+    rc = launch_fullA(p->L2, dim3(ch.nA, (unsigned)batch), st, a, ch.d_A, p->d_band, p->L1,
+                      (int)batch, sW, sD);
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+    rc = launch_fullB(p->L1, dim3(ch.nB, (unsigned)batch), st, a, ch.d_B, p->L2, (int)batch,
+                      sW, sD);
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+  }
+  return SSQK_OK;
+}

@@ -1,0 +1,275 @@
+// Device-side FFT building blocks for gfx950 (CDNA4, wave64).
+//
+// Everything here is compile-time sized: register DFTs of radix 2..16 with
+// constant-folded twiddles, and a workgroup-wide Stockham autosort FFT over
+// rows held in LDS.  Rows are stored with a padded layout so that the strided
+// accesses of the early passes and of the transposed read-out do not pile
+// onto the same LDS banks.
+//
+// Sign convention: DIR = -1 is the forward DFT  X[k] = sum x[n] e^{-2 pi i kn/M},
+// DIR = +1 the unnormalized inverse.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ssqk {
+
+// ------------------------------------------------------------ complex math
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// --------------------------------------------- compile-time trigonometry
+// Taylor series on [0, pi/4] after octant reduction; exact to double
+// rounding for the 2^k-th roots of unity used as butterfly twiddles.
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double ct_sin_small(double x) {
+  double t = x, s = x;
+  for (int n = 1; n < 14; ++n) { t *= -x * x / ((2 * n) * (2 * n + 1)); s += t; }
+  return s;
+}
+constexpr double ct_cos_small(double x) {
+  double t = 1, s = 1;
+  for (int n = 1; n < 14; ++n) { t *= -x * x / ((2 * n - 1) * (2 * n)); s += t; }
+  return s;
+}
+// cos / sin of a = 2*pi*t/N, reduced to the first octant with exact integer
+// symmetries: a -> 2pi - a, a -> pi - a (= 2pi (N-2t)/(2N)), a -> pi/2 - a.
+constexpr double ct_cos2pi(long t, long N) {
+  t = ((t % N) + N) % N;
+  if (2 * t > N) return ct_cos2pi(N - t, N);
+  if (4 * t > N) return -ct_cos2pi(N - 2 * t, 2 * N);
+  if (8 * t > N) return ct_sin_small(2 * kPi * (double)(N - 4 * t) / (4.0 * N));
+  return ct_cos_small(2 * kPi * (double)t / N);
+}
+constexpr double ct_sin2pi(long t, long N) {
+  t = ((t % N) + N) % N;
+  if (2 * t > N) return -ct_sin2pi(N - t, N);
+  if (4 * t > N) return ct_sin2pi(N - 2 * t, 2 * N);
+  if (8 * t > N) return ct_cos_small(2 * kPi * (double)(N - 4 * t) / (4.0 * N));
+  return ct_sin_small(2 * kPi * (double)t / N);
+}
+
+// v * w_N^(DIR*T) with special cases folded at compile time.
+template <int N, int T, int DIR>
+__device__ __forceinline__ float2 twc(float2 v) {
+  constexpr int t = ((T % N) + N) % N;
+  if constexpr (t == 0) {
+    return v;
+  } else if constexpr (2 * t == N) {
+    return make_float2(-v.x, -v.y);
+  } else if constexpr (4 * t == N) {          // * (DIR i)
+    return DIR > 0 ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
+  } else if constexpr (4 * t == 3 * N) {      // * (-DIR i)
+    return DIR > 0 ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
+  } else {
+    constexpr float c = (float)ct_cos2pi(t, N);
+    constexpr float s = (float)(DIR * ct_sin2pi(t, N));
+    return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+  }
+}
+
+// ----------------------------------------------------- register DFTs
+// In-place DFT of R values in registers, natural order in and out.
+template <int R, int DIR>
+struct RegDFT;
+
+template <int DIR>
+struct RegDFT<1, DIR> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <int DIR>
+struct RegDFT<2, DIR> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int DIR>
+struct RegDFT<4, DIR> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = twc<4, 1, DIR>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+// Composite R = A * B: n = B*n1 + n2, k = k1 + A*k2.
+//   X[k1 + A k2] = sum_n2 w_B^{n2 k2} w_R^{n2 k1} sum_n1 v[B n1 + n2] w_A^{n1 k1}
+template <int A, int B, int DIR>
+struct RegDFTComposite {
+  template <int n2>
+  __device__ __forceinline__ static void stage1(const float2* v, float2 (&u)[A * B]) {
+    if constexpr (n2 < B) {
+      float2 t[A];
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) t[n1] = v[B * n1 + n2];
+      RegDFT<A, DIR>::run(t);
+      tw_row<n2, 0>(t);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) u[n2 * A + k1] = t[k1];
+      stage1<n2 + 1>(v, u);
+    }
+  }
+  template <int n2, int k1>
+  __device__ __forceinline__ static void tw_row(float2* t) {
+    if constexpr (k1 < A) {
+      t[k1] = twc<A * B, n2 * k1, DIR>(t[k1]);
+      tw_row<n2, k1 + 1>(t);
+    }
+  }
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 u[A * B];
+    stage1<0>(v, u);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      float2 t[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) t[n2] = u[n2 * A + k1];
+      RegDFT<B, DIR>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = t[k2];
+    }
+  }
+};
+
+template <int DIR>
+struct RegDFT<8, DIR> {
+  __device__ __forceinline__ static void run(float2* v) { RegDFTComposite<2, 4, DIR>::run(v); }
+};
+template <int DIR>
+struct RegDFT<16, DIR> {
+  __device__ __forceinline__ static void run(float2* v) { RegDFTComposite<4, 4, DIR>::run(v); }
+};
+
+// ------------------------------------------------------- LDS row layout
+// Element m of row r lives at  r*RowStride<M> + m + (m >> 4).  The (m >> 4)
+// skew breaks the stride-16 writes of a radix-16 first pass; the odd extra
+// row padding breaks the column-wise read-out of consecutive rows.
+template <int M>
+struct Lds {
+  static constexpr int kRowStride = M + (M >> 4) + 1;
+  __device__ __forceinline__ static int idx(int row, int m) {
+    return row * kRowStride + m + (m >> 4);
+  }
+  static constexpr int elems(int rows) { return rows * kRowStride; }
+};
+
+// ------------------------------------------------ radix plan for size M
+constexpr int log2c(long n) { return n <= 1 ? 0 : 1 + log2c(n / 2); }
+// radix of pass p for M = 2^m: as many radix-16 passes as possible, the
+// remainder (2, 4 or 8) first.
+template <int M>
+struct Plan {
+  static constexpr int kLog = log2c(M);
+  static constexpr int kRem = kLog % 4;
+  static constexpr int kPasses = kLog / 4 + (kRem ? 1 : 0);
+  static constexpr int radix(int p) { return (kRem && p == 0) ? (1 << kRem) : 16; }
+  static constexpr int ns(int p) {   // product of the radices before pass p
+    int n = 1;
+    for (int q = 0; q < p; ++q) n *= radix(q);
+    return n;
+  }
+};
+
+// One Stockham pass of radix R over ROWS rows of length M in LDS, applied
+// to NCH channels (buffers) at once.  tw[t] = exp(+2 pi i t / M).
+// Each thread loads all of its butterflies' inputs before the barrier and
+// writes after it, so the pass can run in place.
+template <int M, int ROWS, int R, int NS, int DIR, int NT, int NCH>
+__device__ __forceinline__ void stockham_pass(float2* const (&buf)[NCH],
+                                              const float2* __restrict__ tw) {
+  constexpr int BPR = M / R;                 // butterflies per row
+  constexpr int NBF = ROWS * BPR;
+  constexpr int ITER = (NBF + NT - 1) / NT;
+  float2 v[NCH][ITER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int j = tid + it * NT;
+    if ((NBF % NT == 0) || j < NBF) {
+      const int row = j / BPR, jj = j - row * BPR;
+      const int k = jj % NS;
+      float2 w[R];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 t = tw[(r * k) * (M / (NS * R))];
+          w[r] = DIR > 0 ? t : cconj(t);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[c][it][r] = buf[c][Lds<M>::idx(row, jj + r * BPR)];
+        if constexpr (NS > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[c][it][r] = cmul(v[c][it][r], w[r]);
+        }
+        RegDFT<R, DIR>::run(v[c][it]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int j = tid + it * NT;
+    if ((NBF % NT == 0) || j < NBF) {
+      const int row = j / BPR, jj = j - row * BPR;
+      const int k = jj % NS;
+      const int base = (jj - k) * R + k;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[c][Lds<M>::idx(row, base + r * NS)] = v[c][it][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int M, int ROWS, int DIR, int NT, int NCH, int P>
+__device__ __forceinline__ void lds_fft_passes(float2* const (&buf)[NCH],
+                                               const float2* __restrict__ tw) {
+  if constexpr (P < Plan<M>::kPasses) {
+    stockham_pass<M, ROWS, Plan<M>::radix(P), Plan<M>::ns(P), DIR, NT, NCH>(buf, tw);
+    lds_fft_passes<M, ROWS, DIR, NT, NCH, P + 1>(buf, tw);
+  }
+}
+
+// Batched FFT of ROWS rows of length M held in LDS (layout Lds<M>), on NCH
+// buffers.  Must be called by all NT threads; starts and ends with a barrier.
+template <int M, int ROWS, int DIR, int NT, int NCH>
+__device__ __forceinline__ void lds_fft(float2* const (&buf)[NCH],
+                                        const float2* __restrict__ tw) {
+  __syncthreads();
+  lds_fft_passes<M, ROWS, DIR, NT, NCH, 0>(buf, tw);
+}
+
+// ---------------------------------------------------------- exact phases
+// exp(+2 pi i * t / L) for an integer t reduced modulo L (L a power of two):
+// the angle is formed exactly, so the only error is sincospi's.
+__device__ __forceinline__ float2 unit_phase(uint64_t t, uint32_t L) {
+  uint32_t r = (uint32_t)(t & (uint64_t)(L - 1));
+  // map to (-1, 1] turns*2 so the argument of sincospi stays small
+  int32_t s = (r > L / 2) ? (int32_t)r - (int32_t)L : (int32_t)r;
+  float x = (float)s * (2.0f / (float)L);   // exact: L is a power of two
+  float sn, cs;
+  sincospif(x, &sn, &cs);
+  return make_float2(cs, sn);
+}
+
+}  // namespace ssqk

@@ -1,0 +1,258 @@
+"""Synchrosqueezing: frequency grids (host) and the GPU squeeze.
+
+Grid construction restates ssqueezepy/ssqueezing.py:228-310
+(_ssq_freqrange, _compute_associated_frequencies, _exp_fm,
+_get_center_frequency) and the bin parameters of algos.py:44-123, 338-374.
+``ssqueeze`` keeps the reference signature (ssqueezing.py:13-224) and runs the
+phase transform, bin search and ordered accumulation on the GPU.
+"""
+from types import FunctionType
+
+import numpy as np
+
+from . import _lib
+from ._common import (EPS64, NOTE, WARN, _process_fs_and_t, assert_is_one_of,
+                      p2up, pi)
+from ._scales import infer_scaletype, logscale_transition_idx, process_scales
+from ._tensors import as_device_tensor, to_output
+from .wavelets import center_frequency
+
+__all__ = ['ssqueeze', 'bin_params_from_freqs', 'compute_associated_frequencies']
+
+
+def _ssq_freqrange(maprange, dt, N, wavelet, scales, was_padded):
+    if isinstance(maprange, tuple):
+        fm, fM = maprange
+    elif maprange == 'maximal':
+        dT = dt * N
+        fm = 1 / dT
+        fM = 1 / (2 * dt)
+    elif maprange in ('peak', 'energy'):
+        kw = dict(wavelet=wavelet, N=N, maprange=maprange, dt=dt,
+                  was_padded=was_padded)
+        fm = _get_center_frequency(**kw, scale=scales[-1])
+        fM = _get_center_frequency(**kw, scale=scales[0])
+    return fm, fM
+
+
+def compute_associated_frequencies(scales, N, wavelet, ssq_scaletype, maprange,
+                                   was_padded=True, dt=1, transform='cwt'):
+    """Frequencies Tx rows are squeezed onto, ascending
+    (ssqueezing.py:247-291)."""
+    fm, fM = _ssq_freqrange(maprange, dt, N, wavelet, scales, was_padded)
+    na = len(scales)
+    if ssq_scaletype == 'log':
+        ssq_freqs = fm * np.power(fM / fm, np.arange(na) / (na - 1))
+    elif ssq_scaletype == 'log-piecewise':
+        idx = logscale_transition_idx(scales)
+        if idx is None:
+            ssq_freqs = fm * np.power(fM / fm, np.arange(na) / (na - 1))
+        else:
+            f0, f2 = fm, fM
+            f1 = _get_center_frequency(wavelet, N, maprange, dt, scales[idx],
+                                       was_padded)
+            t1 = np.arange(0, na - idx - 1) / (na - 1)
+            t2 = np.arange(na - idx - 1, na) / (na - 1)
+            t1 = np.hstack([t1, t2[0]])
+            with np.errstate(over='ignore', invalid='ignore'):
+                sqf1 = _exp_fm(t1, f0, f1)[:-1]
+                sqf2 = _exp_fm(t2, f1, f2)
+            ssq_freqs = np.hstack([sqf1, sqf2])
+            ssq_idx = logscale_transition_idx(ssq_freqs)
+            if ssq_idx is None:
+                raise Exception("couldn't find logscale transition index of "
+                                "generated `ssq_freqs`; something went wrong")
+            assert (na - ssq_idx) == idx, "{} != {}".format(na - ssq_idx, idx)
+    else:
+        if transform == 'cwt':
+            ssq_freqs = np.linspace(fm, fM, na)
+        else:
+            ssq_freqs = np.linspace(0, .5, na) / dt
+    return ssq_freqs
+
+
+def _exp_fm(t, fmin, fmax):
+    tmin, tmax = t.min(), t.max()
+    a = (fmin**tmax / fmax**tmin) ** (1 / (tmax - tmin))
+    b = fmax**(1 / tmax) * (1 / a)**(1 / tmax)
+    return a * b**t
+
+
+def _get_center_frequency(wavelet, N, maprange, dt, scale, was_padded):
+    if was_padded:
+        N = p2up(N)[0]
+    kw = dict(wavelet=wavelet, N=N, scale=scale, kind=maprange)
+    if maprange == 'energy':
+        kw['force_int'] = True
+    wc = center_frequency(**kw)
+    return wc / (2 * pi) / dt
+
+
+def _ensure_nonzero_nonnegative(name, x, silent=False):
+    if x < EPS64:
+        if not silent:
+            WARN("computed `%s` (%.2e) is below EPS64; will set to " % (name, x)
+                 + "EPS64. Advised to check `ssq_freqs`.")
+        x = EPS64
+    return x
+
+
+def bin_params_from_freqs(ssq_freqs, logscale, flipud, omax):
+    """C-ABI bin parameters for ``ssq_freqs`` (algos.py:83-90, 356-374)."""
+    v = np.asarray(ssq_freqs)
+    if logscale:
+        idx = logscale_transition_idx(v)
+        vlmin = float(np.log2(v[0]))
+        if idx is None:
+            dvl = float(np.log2(v[1]) - np.log2(v[0]))
+            dvl = _ensure_nonzero_nonnegative('dvl', dvl)
+            return _lib.bin_params(_lib.BIN_LOG, flipud, omax,
+                                   dict(vlmin=vlmin, dvl=dvl))
+        vlmin1 = float(np.log2(v[idx - 1]))
+        dvl0 = float(np.log2(v[1]) - np.log2(v[0]))
+        dvl1 = float(np.log2(v[idx]) - np.log2(v[idx - 1]))
+        dvl0 = _ensure_nonzero_nonnegative('dvl0', dvl0, silent=True)
+        dvl1 = _ensure_nonzero_nonnegative('dvl1', dvl1)
+        return _lib.bin_params(_lib.BIN_LOG_PIECEWISE, flipud, omax,
+                               dict(vlmin0=vlmin, vlmin1=vlmin1, dvl0=dvl0,
+                                    dvl1=dvl1, idx1=idx - 1))
+    dv = float(v[1] - v[0])
+    dv = _ensure_nonzero_nonnegative('dv', dv)
+    return _lib.bin_params(_lib.BIN_LINEAR, flipud, omax,
+                           dict(vmin=float(v[0]), dv=dv))
+
+
+def cwt_const(scales, cwt_scaletype, nv, na):
+    """Per-scale squeeze weight: ln2/nv for log scales, (s1-s0)/s for linear
+    (ssqueezing.py:124-130)."""
+    if cwt_scaletype.startswith('log'):
+        const = np.log(2) / np.asarray(nv, dtype=np.float64)
+    else:
+        s = np.asarray(scales, dtype=np.float64).ravel()
+        const = (s[1] - s[0]) / s
+    const = np.broadcast_to(np.asarray(const, dtype=np.float64).ravel(), (na,))
+    return np.ascontiguousarray(const)
+
+
+def _apply_squeezing(Wx, squeezing):
+    import torch
+    if isinstance(squeezing, FunctionType):
+        return squeezing(Wx)
+    if squeezing == 'lebesgue':
+        return torch.ones_like(Wx) / Wx.shape[-2]
+    if squeezing == 'abs':
+        return torch.abs(Wx).to(Wx.dtype)
+    return Wx
+
+
+def _check_ssqueezing_args(squeezing, maprange=None, wavelet=None, difftype=None,
+                           difforder=None, get_w=None, transform='cwt'):
+    """Argument validation of ssqueezing.py:314-368 (GPU semantics: only
+    difftype='trig')."""
+    if transform not in ('cwt', 'stft'):
+        raise ValueError("`transform` must be one of: cwt, stft "
+                         "(got %s)" % squeezing)
+    if not isinstance(squeezing, (str, FunctionType)):
+        raise TypeError("`squeezing` must be string or function "
+                        "(got %s)" % type(squeezing))
+    elif isinstance(squeezing, str):
+        assert_is_one_of(squeezing, 'squeezing', ('sum', 'lebesgue', 'abs'))
+    if maprange is not None:
+        if isinstance(maprange, (tuple, list)):
+            if not all(isinstance(m, (float, int)) for m in maprange):
+                raise ValueError("all elements of `maprange` must be "
+                                 "float or int")
+        elif isinstance(maprange, str):
+            assert_is_one_of(maprange, 'maprange', ('maximal', 'peak', 'energy'))
+        else:
+            raise TypeError("`maprange` must be str, tuple, or list "
+                            "(got %s)" % type(maprange))
+        if isinstance(maprange, str) and maprange != 'maximal':
+            if transform != 'cwt':
+                NOTE("string `maprange` currently only functional with "
+                     "`transform='cwt'`")
+            elif wavelet is None:
+                raise ValueError(f"maprange='{maprange}' requires `wavelet`")
+    if difftype is not None:
+        if difftype not in ('trig', 'phase', 'numeric'):
+            raise ValueError("`difftype` must be one of: direct, phase, numeric"
+                             " (got %s)" % difftype)
+        elif difftype != 'trig':
+            raise ValueError("GPU computation only supports `difftype = 'trig'`")
+    if difforder is not None:
+        WARN("`difforder` is ignored if `difftype != 'numeric'`")
+    return difforder
+
+
+def ssqueeze(Wx, w=None, ssq_freqs=None, scales=None, Sfs=None, fs=None, t=None,
+             squeezing='sum', maprange='maximal', wavelet=None, gamma=None,
+             was_padded=True, flipud=False, dWx=None, transform='cwt'):
+    """Synchrosqueeze a CWT or STFT given ``w`` or (``dWx``, ``gamma``)
+    (ssqueezing.py:13-224).  Returns (Tx, ssq_freqs); Tx on the GPU."""
+    import torch
+    if w is None and (dWx is None or gamma is None):
+        raise ValueError("if `w` is None, `dWx` and `gamma` must not be.")
+    _check_ssqueezing_args(squeezing, maprange, transform=transform,
+                           wavelet=wavelet)
+    if scales is None and transform == 'cwt':
+        raise ValueError("`scales` can't be None if `transform == 'cwt'`")
+    Wx = as_device_tensor(Wx, complex_=True)
+    device = Wx.device
+    if w is not None:
+        w = as_device_tensor(w, like=Wx, real=True)
+        if bool((w < 0).any()):
+            raise ValueError("found negatives in `w`")
+    if dWx is not None:
+        dWx = as_device_tensor(dWx, like=Wx, complex_=True)
+    N = Wx.shape[-1]
+    dt, *_ = _process_fs_and_t(fs, t, N)
+
+    if transform == 'cwt':
+        scales, cwt_scaletype, _, nv = process_scales(scales, N, get_params=True)
+    else:
+        cwt_scaletype, nv = None, None
+
+    if not (isinstance(ssq_freqs, np.ndarray) or hasattr(ssq_freqs, 'detach')):
+        ssq_scaletype = ssq_freqs if isinstance(ssq_freqs, str) else cwt_scaletype
+        if ((maprange == 'maximal' or isinstance(maprange, tuple)) and
+                ssq_scaletype == 'log-piecewise'):
+            raise ValueError("can't have `ssq_scaletype = log-piecewise` or "
+                             "tuple with `maprange = 'maximal'` "
+                             "(got %s)" % str(maprange))
+        ssq_freqs = compute_associated_frequencies(
+            scales, N, wavelet, ssq_scaletype, maprange, was_padded, dt,
+            transform)
+    elif transform == 'stft':
+        ssq_scaletype = 'linear'
+        ssq_freqs = np.asarray(ssq_freqs.detach().cpu() if hasattr(
+            ssq_freqs, 'detach') else ssq_freqs)
+    else:
+        ssq_freqs = np.asarray(ssq_freqs.detach().cpu() if hasattr(
+            ssq_freqs, 'detach') else ssq_freqs)
+        ssq_scaletype, _ = infer_scaletype(ssq_freqs)
+
+    na = Wx.shape[-2]
+    if transform == 'cwt':
+        const = cwt_const(scales, cwt_scaletype, nv, na)
+    else:
+        const = np.full(na, float(ssq_freqs[1] - ssq_freqs[0]))
+    bp = bin_params_from_freqs(ssq_freqs, ssq_scaletype.startswith('log'),
+                               flipud, na - 1)
+    bins = torch.empty(Wx.shape, dtype=torch.int16, device=device)
+    if w is None:
+        Sfs_t = None
+        if transform == 'stft':
+            Sfs_t = torch.as_tensor(np.asarray(
+                Sfs.detach().cpu() if hasattr(Sfs, 'detach') else Sfs),
+                dtype=torch.float32, device=device)
+        _lib.bins_from_dwx(Wx, dWx, Sfs_t, bp, gamma, bins)
+    else:
+        _lib.bins_from_w(w, bp, bins)
+    Wsq = _apply_squeezing(Wx, squeezing).contiguous()
+    Tx = torch.empty_like(Wx)
+    cst = torch.as_tensor(const, dtype=torch.float32, device=device)
+    _lib.squeeze(Wsq, bins, cst, na, Tx)
+
+    if (transform == 'cwt' and not flipud) or flipud:
+        ssq_freqs = ssq_freqs[::-1]
+    return Tx, ssq_freqs

@@ -1,0 +1,401 @@
+"""Frequency-domain wavelets: host description plus on-device generation.
+
+``Wavelet`` mirrors the reference class (ssqueezepy/wavelets.py:14-470) for the
+two wavelets on the accelerated path: Morlet (wavelets.py:498-530) and the
+bandpass (L1) generalized Morse wavelet of order 0 (_gmw.py:22-131,187-225).
+
+Host evaluation (``fn``) keeps the reference's dtype semantics exactly, because
+the scale and frequency grids are built from argmax/threshold searches over
+these samples and must come out identical.  The transform itself samples
+``psih`` on the GPU (``ssqk_psih`` and inside the CWT kernels) in float64
+arithmetic rounded to the transform dtype, which is closer to the float64
+result than the reference's float32 evaluation (whose GMW exponent cancels
+terms of magnitude ~100, _gmw.py:203-210).
+"""
+import numpy as np
+from .configs import gdefaults
+from ._common import pi, NOTE, WARN, assert_is_one_of
+
+__all__ = ['Wavelet', 'morlet', 'gmw', 'center_frequency', 'find_maximum',
+           'find_first_occurrence', 'morsefreq', 'aifftshift', 'afftshift']
+
+KIND_GMW = 0
+KIND_MORLET = 1
+
+
+def _xifn(scale, N, dtype=np.float64):
+    """Radian frequencies of an N-point DFT scaled by ``scale``:
+    [0, 1, ..., N//2, -(N - N//2 - 1), ..., -1] * 2pi/N (wavelets.py:473-484).
+    The Nyquist bin of even N is the positive +pi."""
+    h = scale * (2 * pi) / N
+    i = np.arange(N, dtype=np.float64)
+    i[N // 2 + 1:] -= N
+    return (i * h).astype(dtype)
+
+
+def morsefreq(gamma, beta):
+    """Peak (modal) radian frequency of the GMW, (beta/gamma)**(1/gamma)
+    (_gmw.py:611-657, first output)."""
+    return (beta / gamma)**(1 / gamma)
+
+
+class _WaveletFn:
+    """Callable psih(w) with the reference's dtype semantics plus the float64
+    parameters handed to the device kernels."""
+
+    def __init__(self, kind, params64, dtype, fn_name):
+        self.kind = kind
+        self.params64 = params64
+        self.dtype = dtype
+        self.__qualname__ = fn_name
+
+    def __call__(self, w):
+        dt = np.dtype(self.dtype)
+        w = np.atleast_1d(np.array(w, dtype=dt))
+        p = {k: dt.type(v) for k, v in self.params64.items()}
+        if self.kind == KIND_MORLET:
+            # wavelets.py:521-523
+            return p['C1'] * (np.exp(p['C0'] * (w - p['mu'])**2)
+                              - p['ks'] * np.exp(p['C0'] * w**2))
+        # _gmw.py:203-210
+        w_nonneg = (w >= 0)
+        w *= w_nonneg
+        with np.errstate(divide='ignore'):
+            return 2 * np.exp(- p['beta'] * p['wcl'] + p['wc']**p['gamma']
+                              + p['beta'] * np.log(w) - w**p['gamma']
+                              ) * w_nonneg
+
+
+def morlet(mu=None, dtype=None):
+    """Morlet psih(w) = C1 (exp(-(w-mu)^2/2) - ks exp(-w^2/2)) with
+    cs = (1 + e^{-mu^2} - 2 e^{-3mu^2/4})^{-1/2}, ks = e^{-mu^2/2},
+    C1 = sqrt(2) cs pi^{1/4} (wavelets.py:498-520)."""
+    kw = gdefaults('wavelets.morlet', mu=mu, dtype=dtype)
+    mu, dtype = float(kw['mu']), kw['dtype']
+    cs = (1 + np.exp(-mu**2) - 2 * np.exp(-3/4 * mu**2)) ** (-.5)
+    ks = np.exp(-.5 * mu**2)
+    params = dict(mu=mu, ks=ks, C0=-.5, C1=np.sqrt(2) * cs * pi**.25)
+    return _WaveletFn(KIND_MORLET, params, dtype, 'morlet.<locals>.<lambda>')
+
+
+def gmw(gamma=None, beta=None, norm=None, order=None, centered_scale=None,
+        dtype=None):
+    """Bandpass-normalized generalized Morse wavelet of order 0:
+    psih(w) = 2 exp(-beta ln wc + wc^gamma + beta ln w - w^gamma) for w >= 0,
+    0 otherwise, wc = (beta/gamma)^(1/gamma) (_gmw.py:187-210)."""
+    kw = gdefaults('_gmw.gmw', gamma=gamma, beta=beta, norm=norm, order=order,
+                   centered_scale=centered_scale, dtype=dtype)
+    gamma, beta = float(kw['gamma']), float(kw['beta'])
+    if gamma <= 0:
+        raise ValueError(f"`gamma` must be positive (got {gamma})")
+    if beta <= 0:
+        raise ValueError(f"`beta` must be positive (got {beta})")
+    if kw['norm'] != 'bandpass':
+        raise NotImplementedError("only norm='bandpass' GMW is accelerated "
+                                  "(got norm=%s)" % kw['norm'])
+    if kw['order'] not in (0, None):
+        raise NotImplementedError("only order=0 GMW is accelerated")
+    if kw['centered_scale']:
+        raise NotImplementedError("centered_scale=True is not supported")
+    wc = morsefreq(gamma, beta)
+    params = dict(gamma=gamma, beta=beta, wc=wc, wcl=np.log(wc))
+    return _WaveletFn(KIND_GMW, params, kw['dtype'], 'gmw_l1.<locals>.<lambda>')
+
+
+class Wavelet:
+    """Frequency-domain wavelet sampled on the DFT grid ``scale * xi``.
+
+    ``Wavelet('gmw')``, ``Wavelet(('gmw', {'beta': 30}))``,
+    ``Wavelet(('morlet', {'mu': 6}))`` as in the reference
+    (wavelets.py:14-84, :403-470).
+    """
+    SUPPORTED = {'gmw', 'morlet'}
+    DTYPES = {'float32', 'float64'}
+
+    def __init__(self, wavelet='gmw', N=1024, dtype=None):
+        if dtype is not None:
+            dtype = np.dtype(dtype).name
+            assert_is_one_of(dtype, 'dtype', Wavelet.DTYPES)
+        self._dtype = dtype
+        self._validate_and_set_wavelet(wavelet)
+        self.N = N
+
+    @classmethod
+    def _init_if_not_isinstance(cls, wavelet, **kw):
+        if isinstance(wavelet, Wavelet):
+            return wavelet
+        return Wavelet(wavelet, **kw)
+
+    def _validate_and_set_wavelet(self, wavelet):
+        errmsg = ("`wavelet` must be one of: (1) string name of supported "
+                  "wavelet; (2) tuple of (1) and dict of wavelet parameters "
+                  "(e.g. {'mu': 5}). (got: %s)" % str(wavelet))
+        if isinstance(wavelet, tuple):
+            if not (len(wavelet) == 2 and isinstance(wavelet[1], dict)):
+                raise TypeError(errmsg)
+            name, opts = wavelet[0], dict(wavelet[1])
+        elif isinstance(wavelet, str):
+            name, opts = wavelet, {}
+        else:
+            raise TypeError(errmsg)
+        name = name.lower()
+        if name in ('bump', 'cmhat', 'hhhat'):
+            raise NotImplementedError(
+                f"wavelet '{name}' is not on the accelerated path; supported: "
+                f"{sorted(Wavelet.SUPPORTED)}")
+        assert_is_one_of(name, 'wavelet', Wavelet.SUPPORTED)
+        if self._dtype is not None:
+            opts['dtype'] = self._dtype
+        if name == 'gmw':
+            opts = gdefaults('_gmw.gmw', get_all=True, **opts)
+            if opts.get('norm') == 'energy':
+                raise NotImplementedError("GMW norm='energy' (L2) is not "
+                                          "accelerated; use 'bandpass'")
+            self.fn = gmw(**opts)
+        else:
+            opts = gdefaults('wavelets.morlet', get_all=True, **opts)
+            self.fn = morlet(**opts)
+        self._dtype = np.dtype(opts['dtype']).name
+        self.config = opts
+        self.name_short = name
+
+    # ------------------------------------------------------------- basics
+    @property
+    def dtype(self):
+        return self._dtype
+
+    @property
+    def N(self):
+        return self._N
+
+    @N.setter
+    def N(self, value):
+        self._N = int(value)
+        self._xi = _xifn(1, self._N, dtype=np.dtype(self.dtype))
+
+    @property
+    def xi(self):
+        return self._xi
+
+    @property
+    def name(self):
+        return 'GMW L1' if self.name_short == 'gmw' else 'Morlet'
+
+    @property
+    def kind(self):
+        return self.fn.kind
+
+    @property
+    def params64(self):
+        """Float64 parameters in the order the device kernels expect:
+        GMW: (gamma, beta, wc, wcl); Morlet: (mu, ks, C0, C1)."""
+        p = self.fn.params64
+        if self.kind == KIND_GMW:
+            return (p['gamma'], p['beta'], p['wc'], p['wcl'])
+        return (p['mu'], p['ks'], p['C0'], p['C1'])
+
+    def psih64(self, w):
+        """psih(w) evaluated in float64 (used for band limits and tests)."""
+        w = np.asarray(w, dtype=np.float64)
+        p = self.fn.params64
+        if self.kind == KIND_MORLET:
+            return p['C1'] * (np.exp(p['C0'] * (w - p['mu'])**2)
+                              - p['ks'] * np.exp(p['C0'] * w**2))
+        out = np.zeros_like(w)
+        m = w > 0
+        lw = np.log(w[m])
+        out[m] = 2 * np.exp(p['beta'] * (lw - p['wcl'])
+                            - (np.exp(p['gamma'] * lw) - p['wc']**p['gamma']))
+        return out
+
+    def xifn(self, scale=None, N=None):
+        """``scale * xi`` (wavelets.py:111-133)."""
+        if scale is None:
+            scale = 1.
+        scale = np.asarray(scale, dtype=self.dtype)
+        if scale.ndim == 1 and len(scale) > 1:
+            scale = scale.reshape(-1, 1)
+        if N is None:
+            return scale * self.xi
+        return scale * _xifn(1., N, dtype=np.dtype(self.dtype))
+
+    def __call__(self, w=None, *, scale=None, N=None, nohalf=True):
+        """Host psih: ``fn(w)`` or ``fn(scale * xi)`` with optional Nyquist
+        halving (wavelets.py:62-95)."""
+        if w is not None:
+            psih = self.fn(np.asarray(w, dtype=self.dtype))
+        else:
+            psih = self.fn(self.xifn(scale, N))
+        if not nohalf:
+            psih = self._halve_nyquist(psih)
+        return psih
+
+    @staticmethod
+    def _halve_nyquist(psih):
+        N = psih.shape[-1]
+        if N % 2 == 0:
+            psih[..., N // 2] /= 2
+        return psih
+
+    def Psih(self, scale, N=None, nohalf=True, device='cuda'):
+        """psih of shape (len(scale), N) generated on the GPU by the
+        ``ssqk_psih`` kernel; cached per (scale, N, nohalf) like the
+        reference's cache (wavelets.py:135-160)."""
+        from . import _lib
+        import torch
+        N = int(N or self.N)
+        scale = np.asarray(scale, dtype=np.float64).ravel()
+        key = (N, bool(nohalf), scale.tobytes())
+        cache = getattr(self, '_psih_cache', None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        dt = torch.float32 if self.dtype == 'float32' else torch.float64
+        out = torch.empty((len(scale), N), dtype=dt, device=device)
+        _lib.psih(self, scale, N, nohalf, out)
+        self._psih_cache = (key, out)
+        return out
+
+    def band_limits(self, rel=1e-9):
+        """Interval [w_lo, w_hi] of radian argument outside which
+        psih(w) < rel * max psih (float64).  Used to restrict the inverse
+        transform of each scale to the bins where the product can be
+        non-negligible."""
+        cache = getattr(self, '_band_cache', {})
+        if rel in cache:
+            return cache[rel]
+        if self.kind == KIND_GMW:
+            wc = self.fn.params64['wc']
+        else:
+            wc = self.fn.params64['mu']
+        peak = self.psih64(np.array([wc]))[0]
+        th = rel * abs(peak)
+
+        def edge(lo, hi, inside_hi):
+            # bisection for the crossing of |psih| = th between lo and hi
+            for _ in range(200):
+                mid = .5 * (lo + hi)
+                v = abs(self.psih64(np.array([mid]))[0])
+                if (v > th) == inside_hi:
+                    hi = mid
+                else:
+                    lo = mid
+            return lo if inside_hi else hi
+        w_lo = edge(1e-12, wc, True) if self.psih64(np.array([1e-12]))[0] < th else 0.
+        hi = 2 * wc + 10
+        while abs(self.psih64(np.array([hi]))[0]) > th:
+            hi *= 2
+        w_hi = edge(wc, hi, False)
+        cache[rel] = (max(w_lo * (1 - 1e-9), 0.), w_hi * (1 + 1e-9))
+        self._band_cache = cache
+        return cache[rel]
+
+
+#### center frequency & searches ############################################
+def find_maximum(fn, step_size=1e-3, steps_per_search=1e4, step_start=0,
+                 step_limit=1000, min_value=-1):
+    """Max of a unimodal |fn| by blockwise grid search (algos.py:625-662)."""
+    steps_per_search = int(steps_per_search)
+    largest_max = min_value
+    increment = int(steps_per_search * step_size)
+    output_values = -1 * np.ones(steps_per_search)
+    search_idx = 0
+    while True:
+        start = step_start + increment * search_idx
+        end = start + increment
+        input_values = np.linspace(start, end, steps_per_search, endpoint=False)
+        output_values[:] = np.abs(np.asarray(fn(input_values)))
+        output_max = output_values.max()
+        if output_max > largest_max:
+            largest_max = output_max
+            input_value = input_values[np.argmax(output_values)]
+        elif output_max < largest_max:
+            break
+        search_idx += 1
+        if input_values.max() > step_limit:
+            raise ValueError("could not find function maximum")
+    return input_value, largest_max
+
+
+def find_first_occurrence(fn, value, step_size=1e-3, steps_per_search=1e4,
+                          step_start=0, step_limit=1000):
+    """First input where |fn| reaches ``value`` (algos.py:665-703)."""
+    steps_per_search = int(steps_per_search)
+    increment = int(steps_per_search * step_size)
+    output_values = -1 * np.ones(steps_per_search)
+    step_limit_exceeded = False
+    search_idx = 0
+    while True:
+        start = step_start + increment * search_idx
+        end = start + increment
+        input_values = np.linspace(start, end, steps_per_search, endpoint=False)
+        if input_values.max() > step_limit:
+            step_limit_exceeded = True
+            input_values = np.clip(input_values, None, step_limit)
+        output_values[:] = np.abs(np.asarray(fn(input_values)))
+        mxdiff = np.abs(np.diff(output_values)).max()
+        if np.any(np.abs(output_values - value) <= mxdiff):
+            idx = np.argmin(np.abs(output_values - value))
+            break
+        search_idx += 1
+        if step_limit_exceeded:
+            raise ValueError("could not find input value to yield function "
+                             f"output value={value}")
+    return input_values[idx], output_values[idx]
+
+
+def afftshift(xh):
+    """Move the right N//2+1 bins to the left (wavelets.py:932-951)."""
+    N = len(xh)
+    if N % 2 == 0:
+        return np.concatenate([xh[N // 2 - 1:], xh[:N // 2 - 1]])
+    return np.fft.fftshift(xh)
+
+
+def aifftshift(xh):
+    """Inverse of ``afftshift`` (wavelets.py:954-969): xhs[i + N//2 - 1] =
+    xh[i] for i <= N//2, xhs[i - N//2 - 1] = xh[i] otherwise."""
+    N = len(xh)
+    if N % 2 == 0:
+        out = np.zeros(N, dtype=xh.dtype)
+        out[N // 2 - 1:] = xh[:N // 2 + 1]
+        out[:N // 2 - 1] = xh[N // 2 + 1:]
+        return out
+    return np.fft.ifftshift(xh)
+
+
+def center_frequency(wavelet, scale=None, N=1024, kind='energy',
+                     force_int=None, viz=False):
+    """Radian center frequency: 'peak' (discrete argmax of |psih(scale*w)|^2
+    on the N-point grid), 'peak-ct' (continuous peak) or 'energy'
+    (wavelets.py:611-750)."""
+    def _params(wavelet, scale, N):
+        w = aifftshift(_xifn(1, N))
+        psih = np.asarray(wavelet(np.asarray(scale) * w))
+        apsih2 = np.abs(psih)**2
+        return w, psih, apsih2
+
+    def _peak_ct_wc(wavelet, N):
+        wc, _ = find_maximum(wavelet.fn)
+        return float(wc)
+
+    assert_is_one_of(kind, 'kind', ('energy', 'peak', 'peak-ct'))
+    if scale is None and kind != 'peak-ct':
+        wc, _ = find_maximum(wavelet.fn)
+        scale = (4 / pi) * wc
+    wavelet = Wavelet._init_if_not_isinstance(wavelet)
+    if kind == 'energy':
+        from scipy import integrate
+        force_int = force_int or True
+        use_formula = not force_int
+        if use_formula:
+            scale_orig = scale
+            scale = (4 / pi) * _peak_ct_wc(wavelet, N)
+        w, psih, apsih2 = _params(wavelet, scale, N)
+        wc = integrate.trapezoid(apsih2 * w) / integrate.trapezoid(apsih2)
+        if use_formula:
+            wc *= (scale / scale_orig)
+        return float(wc)
+    elif kind == 'peak':
+        w, psih, apsih2 = _params(wavelet, scale, N)
+        return float(w[np.argmax(apsih2)])
+    return _peak_ct_wc(wavelet, N)

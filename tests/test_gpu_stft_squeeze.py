@@ -1,0 +1,96 @@
+"""GPU parity of the STFT / ssq_stft path and of the standalone squeeze
+entry points against the float64 oracle and the reference goldens."""
+import numpy as np
+import pytest
+import scipy.signal as sig
+
+from conftest import rel_l2
+from oracle import ssq_oracle as O
+
+pytestmark = pytest.mark.gpu
+EPS32 = np.finfo(np.float32).eps
+
+
+def _dwin(win):
+    n = len(win)
+    xi = O.xi_grid(n)
+    if n % 2 == 0:
+        xi[n // 2] = 0
+    return np.fft.ifft(np.fft.fft(win) * 1j * xi).real
+
+
+@pytest.mark.parametrize('N,n_fft,hop', [(100, 64, 1), (1000, 128, 3), (20000, 512, 1)])
+def test_stft_vs_oracle(N, n_fft, hop):
+    import ssqueezepy_amd as sq
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(N).astype(np.float32)
+    Sx, dSx = sq.stft(x, 'hann', n_fft=n_fft, hop_len=hop, derivative=True, fs=2.)
+    win = sig.get_window('hann', n_fft, fftbins=True)
+    Sx_o, dSx_o = O.stft(x.astype(np.float64), win, _dwin(win), n_fft, hop, fs=2.)
+    assert Sx.shape == Sx_o.shape
+    assert rel_l2(Sx.cpu().numpy(), Sx_o) < 1e-5
+    assert rel_l2(dSx.cpu().numpy(), dSx_o) < 1e-5
+
+
+@pytest.mark.parametrize('N,n_fft', [(100, 64), (5000, 256), (30000, 512)])
+def test_ssq_stft_vs_oracle(N, n_fft):
+    import ssqueezepy_amd as sq
+    n = np.arange(N)
+    rng = np.random.default_rng(1)
+    x = (np.cos(2 * np.pi * (0.02 * n + 0.09 * n**2 / N)) + 0.5 * np.cos(2 * np.pi * .3 * n)
+         + 0.1 * rng.standard_normal(N)).astype(np.float32)
+    Tx, Sx, f, Sfs = sq.ssq_stft(x, 'hann', n_fft=n_fft)
+    win = sig.get_window('hann', n_fft, fftbins=True)
+    Tx_o, Sx_o, *_ = O.ssq_stft(x.astype(np.float64), win, _dwin(win), n_fft, 10 * EPS32)
+    assert rel_l2(Sx.cpu().numpy(), Sx_o) < 1e-5
+    et = rel_l2(Tx.cpu().numpy(), Tx_o)
+    print(f"ssq_stft N={N} n_fft={n_fft}: Tx rel-L2 {et:.2e}", flush=True)
+    assert et < 1e-4
+
+
+def test_ssq_stft_vs_reference_golden(golden):
+    import ssqueezepy_amd as sq
+    e = golden('e2e_stft')
+    for key, kw in [('hann64_64', dict(window='hann', n_fft=64)),
+                    ('hann32_fs', dict(window='hann', n_fft=32, fs=2., flipud=True))]:
+        Tx, Sx, f, Sfs, w, dSx = sq.ssq_stft(e['x'], get_w=True, get_dWx=True, **kw)
+        assert rel_l2(Sx.cpu().numpy(), e[key + '/Sx']) < 1e-5
+        assert rel_l2(dSx.cpu().numpy(), e[key + '/dSx']) < 1e-5
+        assert rel_l2(Tx.cpu().numpy(), e[key + '/Tx']) < 1e-4
+        assert np.allclose(f, e[key + '/ssq_freqs'])
+
+
+@pytest.mark.parametrize('stype', ['log-piecewise', 'log', 'linear'])
+@pytest.mark.parametrize('flip', [0, 1])
+def test_squeeze_entry_points_vs_reference(golden, stype, flip):
+    from ssqueezepy_amd import algos
+    s = golden('squeeze')
+    Wx = s['Wx'].astype(np.complex64)
+    dWx = s['dWx'].astype(np.complex64)
+    w = s['w'].astype(np.float32)
+    f = s[stype + '/ssq_freqs']
+    log = stype.startswith('log')
+    T1 = algos.indexed_sum_onfly(Wx, w, f, s['const'], log, bool(flip)).cpu().numpy()
+    assert rel_l2(T1, s[f'{stype}/{flip}/onfly']) < 1e-5
+    T2 = algos.ssqueeze_fast(Wx, dWx, f / len(Wx) / 4, s['const'], log, bool(flip),
+                             gamma=1e-2).cpu().numpy()
+    assert rel_l2(T2, s[f'{stype}/{flip}/fast']) < 1e-5
+
+
+def test_phase_entry_points_vs_reference(golden):
+    from ssqueezepy_amd import algos
+    s = golden('squeeze')
+    Wx = s['Wx'].astype(np.complex64)
+    dWx = s['dWx'].astype(np.complex64)
+    w = algos.phase_cwt_gpu(Wx, dWx, 1e-2).cpu().numpy()
+    ref = s['phase_cwt']
+    fin = np.isfinite(ref)
+    assert np.array_equal(fin, np.isfinite(w))
+    assert np.allclose(w[fin], ref[fin], rtol=1e-5)
+    w = algos.phase_stft_gpu(Wx, dWx, s['stft/Sfs'], 1e-2).cpu().numpy()
+    ref = s['phase_stft']
+    fin = np.isfinite(ref)
+    assert np.allclose(w[fin], ref[fin], rtol=1e-5, atol=1e-7)
+    T = algos.ssqueeze_fast(Wx, dWx, s['stft/Sfs'], s['const'], False, False, gamma=1e-2,
+                            Sfs=s['stft/Sfs']).cpu().numpy()
+    assert rel_l2(T, s['stft/fast']) < 1e-5

@@ -1,0 +1,79 @@
+"""The product's host-side grids (scales, ssq_freqs, squeeze weights, bin
+parameters) against the reference's own values (golden grids.npz)."""
+import ast
+
+import numpy as np
+import pytest
+
+from ssqueezepy_amd import Wavelet
+from ssqueezepy_amd._common import p2up, padsignal_np, reflect_index
+from ssqueezepy_amd._scales import process_scales
+from ssqueezepy_amd.ssqueezing import compute_associated_frequencies
+
+import conftest
+
+_G = np.load(conftest.GOLDEN + '/grids.npz')
+KEYS = sorted({k.split('/')[0] for k in _G.files})
+
+
+@pytest.mark.parametrize('key', KEYS)
+def test_grid_matches_reference(key):
+    g = lambda name: _G[key + '/' + name]
+    scaletype, st, maprange, dt, wname, opts = [str(v) for v in g('info')]
+    N, na, nv = [int(v) for v in g('meta')]
+    wav = Wavelet((wname, dict(ast.literal_eval(opts))))
+    s64 = process_scales(scaletype, N, wav, nv=nv)
+    assert len(s64) == na
+    assert np.array_equal(s64.ravel(), g('scales64'))
+    s_dt = s64.astype(dt)
+    s_p, cwt_st, _, nvs = process_scales(s_dt, N, get_params=True)
+    f = compute_associated_frequencies(s_p, N, wav, st, maprange, True, 1, 'cwt')
+    assert np.array_equal(np.asarray(f), g('ssq_freqs'))
+    if cwt_st.startswith('log'):
+        assert np.array_equal(np.asarray(nvs, dtype=np.float64).ravel(), g('nv'))
+    for name in ('vlmin', 'dvl', 'vlmin0', 'vlmin1', 'dvl0', 'dvl1', 'idx1', 'vmin', 'dv'):
+        if key + '/p_' + name in _G.files:
+            from ssqueezepy_amd.ssqueezing import bin_params_from_freqs
+            bp = bin_params_from_freqs(f, st.startswith('log'), False, na - 1)
+            val = {'vlmin': bp.v0, 'dvl': bp.d0, 'vlmin0': bp.v0, 'dvl0': bp.d0,
+                   'vlmin1': bp.v1, 'dvl1': bp.d1, 'idx1': bp.idx1, 'vmin': bp.v0,
+                   'dv': bp.d0}[name]
+            assert val == float(g('p_' + name)), name
+
+
+@pytest.mark.parametrize('key', [k for k in KEYS if '_float32_' in k])
+def test_product_grid_is_the_float64_reference_grid(key):
+    # ssq_cwt on a float32 wavelet squeezes onto the float64 reference's
+    # grid (identical to the float32 one except at N=2^20)
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+    k64 = key.replace('_float32_', '_float64_')
+    scaletype, st, maprange, dt, wname, opts = [str(v) for v in _G[key + '/info']]
+    N, na, nv = [int(v) for v in _G[key + '/meta']]
+    if scaletype == 'linear' or st != ('log-piecewise' if scaletype == 'log-piecewise'
+                                       else st) or maprange != 'peak':
+        pytest.skip('grid variant not produced by ssq_cwt defaults')
+    wav = Wavelet((wname, dict(ast.literal_eval(opts))))
+    g = ssq_cwt_grid(N, wav, scaletype, nv, ssq_freqs=st, maprange=maprange)
+    assert np.array_equal(g['scales64'], _G[k64 + '/scales64'])
+    assert np.array_equal(g['ssq_freqs'], _G[k64 + '/ssq_freqs'])
+    assert np.allclose(g['const'], _G[k64 + '/const'], rtol=1e-15, atol=0)
+    assert g['scales'].dtype == np.float32
+
+
+def test_p2up_and_padding():
+    assert p2up(160000) == (262144, 51072, 51072)
+    assert p2up(10000) == (16384, 3192, 3192)
+    assert p2up(1048576) == (2097152, 524288, 524288)
+    rng = np.random.default_rng(0)
+    for N in (1, 2, 3, 7, 100, 101):
+        x = rng.standard_normal(N)
+        for padlen in (None, N + 511, N + 2 * N + 5):
+            for mode in ('reflect', 'zero', 'symmetric', 'replicate', 'wrap'):
+                if mode == 'reflect' and N == 1:
+                    continue
+                xp, up, n1, n2 = padsignal_np(x, mode, padlen)
+                assert len(xp) == up == n1 + N + n2
+            xp, up, n1, n2 = padsignal_np(x, 'reflect', padlen) if N > 1 else (None,) * 4
+            if N > 1:
+                idx = reflect_index(np.arange(up) - n1, N)
+                assert np.array_equal(xp, x[idx])
