@@ -56,6 +56,36 @@ const float2* twiddles(int M) {
   cache[key] = d;
   return d;
 }
+
+const float2* phase_table(int L) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, float2*> cache;
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_pair(dev, L);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int S = ssqk::phase_tab_split(L);
+  const int n_lo = 1 << S, n_hi = (L >> S) > 0 ? (L >> S) : 1;
+  std::vector<float2> h(n_lo + n_hi);
+  for (int j = 0; j < n_lo; ++j) {
+    const double a = 2.0 * M_PI * (double)j / (double)L;
+    h[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  for (int j = 0; j < n_hi; ++j) {
+    const double a = 2.0 * M_PI * (double)((int64_t)j << S) / (double)L;
+    h[n_lo + j] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * h.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  cache[key] = d;
+  return d;
+}
 }  // namespace ssqk_host
 
 extern "C" const char* ssqk_last_error(void) { return g_err; }

@@ -149,7 +149,18 @@ int check_hip(hipError_t e, const char* what);
 // Twiddle table exp(+2 pi i t / M), t in [0, M), owned by the library for
 // the process lifetime, one per (device, M).
 const float2* twiddles(int M);
+// Two-level phase table of ssqk::phase_tab for length L (power of two).
+const float2* phase_table(int L);
 }  // namespace ssqk_host
+
+// twiddle table of size M into `var`, failing the entry point if it could
+// not be created
+#define SSQK_GET_TW(var, M)                                               \
+  const float2* var = ssqk_host::twiddles(M);                             \
+  if (!var) {                                                             \
+    ssqk_host::set_error("could not create the %d-point twiddle table", (int)(M)); \
+    return SSQK_ERR_HIP;                                                  \
+  }
 
 #define SSQK_CHECK(expr)                                                  \
   do {                                                                    \

@@ -51,6 +51,8 @@ struct CwtArgs {
   float gamma2;
   WavDev wav;
   BinDev bp;
+  const float2* ptab;       // phase table of length L (ssqk::phase_tab)
+  int pS;
 };
 
 // Source index of padded sample n (numpy pad modes, utils/common.py:123-158);
@@ -150,14 +152,14 @@ __global__ __launch_bounds__(kNT) void k_cwt_band(CwtArgs a, const int4* __restr
   // phase ramps w_L^{m (a0 + r)}: rows r >= 1 first, then row 0 in place
   for (int e = threadIdx.x + M; e < ROWS * M; e += kNT) {
     const int r = e / M, m = e - r * M;
-    const float2 ph = unit_phase((uint64_t)m * (uint64_t)(a0 + r), L);
+    const float2 ph = phase_tab((uint64_t)m * (uint64_t)(a0 + r), a.ptab, L, a.pS);
     sW[Lds<M>::idx(r, m)] = cmul(sW[Lds<M>::idx(0, m)], ph);
     sD[Lds<M>::idx(r, m)] = cmul(sD[Lds<M>::idx(0, m)], ph);
   }
   __syncthreads();
   if (a0 != 0) {
     for (int m = threadIdx.x; m < M; m += kNT) {
-      const float2 ph = unit_phase((uint64_t)m * (uint64_t)a0, L);
+      const float2 ph = phase_tab((uint64_t)m * (uint64_t)a0, a.ptab, L, a.pS);
       sW[Lds<M>::idx(0, m)] = cmul(sW[Lds<M>::idx(0, m)], ph);
       sD[Lds<M>::idx(0, m)] = cmul(sD[Lds<M>::idx(0, m)], ph);
     }
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(kNT) void k_cwt_band(CwtArgs a, const int4* __restr
     if (ph_a >= D) continue;
     const int64_t n = ph_a + (int64_t)D * q;
     if (n < a.n1 || n >= (int64_t)a.n1 + a.N) continue;
-    float2 ph = unit_phase((uint64_t)ka * (uint64_t)n, L);
+    float2 ph = phase_tab((uint64_t)ka * (uint64_t)n, a.ptab, L, a.pS);
     ph.x *= invL;
     ph.y *= invL;
     emit(a, b, i, n, cmul(sW[Lds<M>::idx(r, q)], ph), cmul(sD[Lds<M>::idx(r, q)], ph));
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(kNT) void k_cwt_fullA(CwtArgs a, const int4* __rest
   for (int e = threadIdx.x; e < G * L2; e += kNT) {
     const int g = e % G, n2 = e / G;
     const int k1 = k1_0 + g;
-    const float2 ph = unit_phase((uint64_t)k1 * (uint64_t)n2, a.L);
+    const float2 ph = phase_tab((uint64_t)k1 * (uint64_t)n2, a.ptab, a.L, a.pS);
     const int64_t o = (base + n2) * L1 + k1;
     scrW[o] = cmul(sW[Lds<L2>::idx(g, n2)], ph);
     scrD[o] = cmul(sD[Lds<L2>::idx(g, n2)], ph);
@@ -259,6 +261,8 @@ struct FwdArgs {
   int L, pad;
   float2* xh;
   int64_t xh_stride;
+  const float2* ptab;
+  int pS;
 };
 
 template <int L>
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(kNT) void k_fwd_A(FwdArgs f, const float2* __restri
   for (int e = threadIdx.x; e < G * L2; e += kNT) {
     const int g = e % G, n2 = e / G;
     const int k1 = k1_0 + g;
-    const float2 ph = cconj(unit_phase((uint64_t)k1 * (uint64_t)n2, f.L));
+    const float2 ph = cconj(phase_tab((uint64_t)k1 * (uint64_t)n2, f.ptab, f.L, f.pS));
     scr[((int64_t)b * L2 + n2) * L1 + k1] = cmul(s[Lds<L2>::idx(g, n2)], ph);
   }
 }
@@ -326,11 +330,12 @@ __global__ __launch_bounds__(kNT) void k_fwd_B(FwdArgs f, const float2* __restri
   MACRO(512) MACRO(1024) MACRO(2048) MACRO(4096) MACRO(8192)
 
 int launch_band(int M, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks) {
+  SSQK_GET_TW(tw, M);
   switch (M) {
 #define C(m)                                                                       \
   case m:                                                                          \
     hipLaunchKernelGGL(k_cwt_band<m>, grid, dim3(kNT), 0, st, a, tasks,            \
-                       ssqk_host::twiddles(m));                                    \
+                       tw);                                    \
     break;
     SSQK_POW2_CASES(C)
 #undef C
@@ -343,11 +348,12 @@ int launch_band(int M, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* 
 
 int launch_fullA(int L2, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks,
                  const int2* band, int L1, int batch, float2* sW, float2* sD) {
+  SSQK_GET_TW(tw, L2);
   switch (L2) {
 #define C(m)                                                                       \
   case m:                                                                          \
     hipLaunchKernelGGL(k_cwt_fullA<m>, grid, dim3(kNT), 0, st, a, tasks, band,     \
-                       ssqk_host::twiddles(m), L1, batch, sW, sD);                 \
+                       tw, L1, batch, sW, sD);                 \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
@@ -360,11 +366,12 @@ int launch_fullA(int L2, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
 
 int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks,
                  int L2, int batch, const float2* sW, const float2* sD) {
+  SSQK_GET_TW(tw, L1);
   switch (L1) {
 #define C(m)                                                                       \
   case m:                                                                          \
     hipLaunchKernelGGL(k_cwt_fullB<m>, grid, dim3(kNT), 0, st, a, tasks,           \
-                       ssqk_host::twiddles(m), L2, batch, sW, sD);                 \
+                       tw, L2, batch, sW, sD);                 \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
@@ -378,11 +385,12 @@ int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
 int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, float2* scr) {
   const int L = f.L;
   if (L <= kSmallFwd) {
+    SSQK_GET_TW(tw, L);
     switch (L) {
 #define C(m)                                                                       \
   case m:                                                                          \
     hipLaunchKernelGGL(k_fwd_small<m>, dim3(batch), dim3(kNT), 0, st, f,           \
-                       ssqk_host::twiddles(m));                                    \
+                       tw);                                    \
     break;
       SSQK_POW2_CASES(C) C(16384)
 #undef C
@@ -392,11 +400,13 @@ int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, floa
     }
     return SSQK_OK;
   }
+  SSQK_GET_TW(twA, L2);
+  SSQK_GET_TW(twB, L1);
   switch (L2) {
 #define C(m)                                                                       \
   case m:                                                                          \
     hipLaunchKernelGGL(k_fwd_A<m>, dim3(L1 / (kP / m), batch), dim3(kNT), 0, st, f,  \
-                       ssqk_host::twiddles(m), L1, scr);                           \
+                       twA, L1, scr);                           \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
@@ -408,7 +418,7 @@ int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, floa
 #define C(m)                                                                       \
   case m:                                                                          \
     hipLaunchKernelGGL(k_fwd_B<m>, dim3(L2 / (kP / m), batch), dim3(kNT), 0, st, f,  \
-                       ssqk_host::twiddles(m), L2, scr);                           \
+                       twB, L2, scr);                           \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
@@ -685,6 +695,12 @@ extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, voi
   FwdArgs f;
   f.x = x; f.N = p->N; f.n1 = p->n1; f.L = (int)p->L; f.pad = p->pad;
   f.xh = xh; f.xh_stride = xh_stride(p);
+  f.ptab = ssqk_host::phase_table((int)p->L);
+  f.pS = phase_tab_split(p->L);
+  if (!f.ptab) {
+    ssqk_host::set_error("could not create the phase table");
+    return SSQK_ERR_HIP;
+  }
   int rc = launch_fwd(f, (int)batch, st, p->fL1, p->fL2, (float2*)(ws + off_fwd));
   if (rc) return rc;
   SSQK_CHECK(hipGetLastError());
@@ -701,6 +717,8 @@ extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, voi
   a.wav = p->wav;
   ssqk_bin_params zero{};
   a.bp = make_bin_dev(bp ? bp : &zero);
+  a.ptab = f.ptab;
+  a.pS = f.pS;
   for (const auto& c : p->classes) {
     rc = launch_band(c.M, dim3(c.ntasks, (unsigned)batch), st, a, c.d_tasks);
     if (rc) return rc;

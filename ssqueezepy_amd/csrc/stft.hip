@@ -142,12 +142,13 @@ extern "C" int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t 
   ssqk_bin_params zero{};
   a.bp = make_bin_dev(bp ? bp : &zero);
   hipStream_t st = (hipStream_t)stream;
+  SSQK_GET_TW(tw, (int)n_fft);
   switch (n_fft) {
 #define C(m)                                                                          \
   case m:                                                                             \
     hipLaunchKernelGGL(k_stft<m>,                                                     \
                        dim3((unsigned)((a.n_frames + kP / m - 1) / (kP / m)), (unsigned)batch), \
-                       dim3(kNT), 0, st, a, ssqk_host::twiddles(m));                  \
+                       dim3(kNT), 0, st, a, tw);                  \
     break;
     C(2) C(4) C(8) C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096) C(8192)
 #undef C

@@ -131,7 +131,7 @@ def cwt_const(scales, cwt_scaletype, nv, na):
         s = np.asarray(scales, dtype=np.float64).ravel()
         const = (s[1] - s[0]) / s
     const = np.broadcast_to(np.asarray(const, dtype=np.float64).ravel(), (na,))
-    return np.ascontiguousarray(const)
+    return np.array(const, dtype=np.float64, copy=True)
 
 
 def _apply_squeezing(Wx, squeezing):
