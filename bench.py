@@ -106,6 +106,9 @@ def main():
     alg_bytes = 4 * N + 16 * units
     achieved = alg_bytes / (ms_gpu * 1e-3) / 1e9
 
+    from ssqueezepy_amd import _lib
+    plan = list(_lib._plans.values())[-1]
+    paths = dict(zip(('band', 'overlap_save', 'four_step'), plan.paths()))
     res = {
         "metric": "ssq_cwt Gscale*samples/s (N=160k, ~300 scales, fp32)",
         "value": round(value, 3),
@@ -122,7 +125,7 @@ def main():
         "config": {"workload": "ssq_cwt, 1 signal, N=%d, GMW(3,60), nv=32 "
                                "log-piecewise (na=%d), reflect pad L=%d, "
                                "log-piecewise ssq_freqs" % (N, na, 2 * 2**int(np.round(np.log2(N)))),
-                   "batch": 1, "N": N, "na": na},
+                   "batch": 1, "N": N, "na": na, "scales_per_path": paths},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

@@ -78,6 +78,10 @@ int ssqk_cwt_plan_create(ssqk_cwt_plan* plan, int64_t N, int64_t L,
                          const double* wavelet_params, const double* scales,
                          int64_t na, double dt, double band_rel);
 int ssqk_cwt_plan_destroy(ssqk_cwt_plan plan);
+/* Number of scales the plan computes on each path: counts[0] band-limited
+ * inverse FFTs, counts[1] overlap-save convolution, counts[2] full-length
+ * four-step FFTs (no reference counterpart: plan introspection). */
+int ssqk_cwt_plan_paths(ssqk_cwt_plan plan, int64_t* counts);
 /* Device scratch bytes ssqk_cwt_exec needs for `batch` signals. */
 int ssqk_cwt_workspace_size(ssqk_cwt_plan plan, int64_t batch, size_t* bytes);
 

@@ -39,6 +39,7 @@ _SIGS = {
                                      c_int64, c_int, c_int, c_double_p, c_double_p,
                                      c_int64, c_double, c_double]),
     'ssqk_cwt_plan_destroy': (c_int, [c_void_p]),
+    'ssqk_cwt_plan_paths': (c_int, [c_void_p, ctypes.POINTER(c_int64)]),
     'ssqk_cwt_workspace_size': (c_int, [c_void_p, c_int64,
                                         ctypes.POINTER(c_size_t)]),
     'ssqk_cwt_exec': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
@@ -138,6 +139,12 @@ class CwtPlan:
                 kind, p, s.ctypes.data_as(c_double_p), self.na, float(dt),
                 float(band_rel)))
         self._h = handle
+
+    def paths(self):
+        """Scales on the (band, overlap-save, four-step) paths."""
+        c = (c_int64 * 3)()
+        check(load().ssqk_cwt_plan_paths(self._h, c))
+        return tuple(c)
 
     def workspace_bytes(self, batch):
         n = c_size_t()

@@ -161,13 +161,16 @@ __global__ void k_bins_w(const float* __restrict__ w, int64_t total, BinDev bp,
 
 // -------------------------------------------------------------- squeeze
 // One workgroup owns W consecutive columns of one signal for every scale.
-// Wave v handles columns [16v, 16v+16); the four lanes of a column split the
-// output rows into quarters, so every (row, column) accumulator has exactly
-// one owner lane and the per-column sum is formed in ascending scale order.
-// Accumulators live in LDS as separate real/imaginary planes [nf][W].
+// Wave v handles columns [v W/4, (v+1) W/4); the lanes of a column split the
+// output rows into equal slices, so every (row, column) accumulator has
+// exactly one owner lane and each column is summed in ascending scale order
+// (deterministic, no atomics).  The lanes of a column load the same
+// addresses, which the memory pipeline serves once.  Accumulators are
+// interleaved complex [nf][W] in LDS; kSqU scales are loaded ahead so each
+// wave keeps enough HBM requests in flight.
 constexpr int kSqW = 64;
 constexpr int kSqNT = 256;
-constexpr int kSqU = 16;    // scales loaded ahead per lane
+constexpr int kSqU = 32;
 
 __global__ __launch_bounds__(kSqNT) void k_squeeze(const float2* __restrict__ Wx,
                                                    const uint16_t* __restrict__ bins,
@@ -175,48 +178,47 @@ __global__ __launch_bounds__(kSqNT) void k_squeeze(const float2* __restrict__ Wx
                                                    int nf, int64_t N, int64_t in_stride,
                                                    int64_t out_stride, int W,
                                                    float2* __restrict__ Tx) {
-  extern __shared__ float s_acc[];           // [2][nf][W]
-  float* acc_re = s_acc;
-  float* acc_im = s_acc + (size_t)nf * W;
+  extern __shared__ float2 s_acc[];          // [nf][W]
   const int b = blockIdx.y;
   const int64_t j0 = (int64_t)blockIdx.x * W;
   const int tid = threadIdx.x;
-  for (int e = tid; e < 2 * nf * W; e += kSqNT) s_acc[e] = 0.f;
+  for (int e = tid; e < nf * W; e += kSqNT) s_acc[e] = make_float2(0.f, 0.f);
   __syncthreads();
 
-  const int lanes_per_col = kSqNT / W;       // 4 for W = 64
+  const int lanes_per_col = kSqNT / W;
   const int wave = tid >> 6, lane = tid & 63;
-  const int cols_per_wave = W / 4;           // 4 waves
+  const int cols_per_wave = W / 4;
   const int c = wave * cols_per_wave + (lane % cols_per_wave);
-  const int q = lane / cols_per_wave;        // row slice owned by this lane
+  const int q = lane / cols_per_wave;
   const int rlo = (int)(((int64_t)nf * q) / lanes_per_col);
   const int rhi = (int)(((int64_t)nf * (q + 1)) / lanes_per_col);
   const int64_t j = j0 + c;
   const bool live = j < N;
-  const float2* wp = Wx + (int64_t)b * in_stride + j;
-  const uint16_t* bp_ = bins + (int64_t)b * in_stride + j;
+  const float2* wp = Wx + (int64_t)b * in_stride + (live ? j : 0);
+  const uint16_t* bp_ = bins + (int64_t)b * in_stride + (live ? j : 0);
 
   for (int i0 = 0; i0 < na; i0 += kSqU) {
     float2 wv[kSqU];
-    int kv[kSqU];
+    uint32_t kv[kSqU];
 #pragma unroll
     for (int u = 0; u < kSqU; ++u) {
       const int i = i0 + u;
       if (live && i < na) {
-        wv[u] = wp[(int64_t)i * N];
         kv[u] = bp_[(int64_t)i * N];
+        wv[u] = wp[(int64_t)i * N];
       } else {
         kv[u] = kNoBin;
       }
     }
 #pragma unroll
     for (int u = 0; u < kSqU; ++u) {
-      const int k = kv[u];
+      const int k = (int)kv[u];
       if (k >= rlo && k < rhi) {
         const float cc = cst[i0 + u];
-        const int a = k * W + c;
-        acc_re[a] += wv[u].x * cc;
-        acc_im[a] += wv[u].y * cc;
+        float2 acc = s_acc[k * W + c];
+        acc.x += wv[u].x * cc;
+        acc.y += wv[u].y * cc;
+        s_acc[k * W + c] = acc;
       }
     }
   }
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(kSqNT) void k_squeeze(const float2* __restrict__ Wx
   for (int e = tid; e < nf * W; e += kSqNT) {
     const int k = e / W, cc = e - k * W;
     const int64_t jj = j0 + cc;
-    if (jj < N) tp[(int64_t)k * N + jj] = make_float2(acc_re[e], acc_im[e]);
+    if (jj < N) tp[(int64_t)k * N + jj] = s_acc[e];
   }
 }
 
@@ -323,13 +325,13 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
   // accumulators [2][nf][W] in LDS; W = 64 columns while it fits 160 KiB,
   // narrower for very many output rows
   int W = kSqW;
-  while (W > 4 && (size_t)2 * nf * W * sizeof(float) > 160 * 1024) W /= 2;
-  if ((size_t)2 * nf * W * sizeof(float) > 160 * 1024) {
+  while (W > 4 && (size_t)nf * W * sizeof(float2) > 160 * 1024) W /= 2;
+  if ((size_t)nf * W * sizeof(float2) > 160 * 1024) {
     ssqk_host::set_error("nf=%lld output rows do not fit the LDS accumulator",
                          (long long)nf);
     return SSQK_ERR_UNSUPPORTED;
   }
-  const size_t lds = (size_t)2 * nf * W * sizeof(float);
+  const size_t lds = (size_t)nf * W * sizeof(float2);
   static bool attr_set = false;
   if (!attr_set) {
     SSQK_CHECK(hipFuncSetAttribute((const void*)k_squeeze,

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -254,6 +255,121 @@ __global__ __launch_bounds__(kNT) void k_cwt_fullB(CwtArgs a, const int4* __rest
   }
 }
 
+// ------------------------------------------------------- overlap-save path
+// Scales whose wavelet is short in time (half support H samples, H << N)
+// are a linear convolution of the padded signal with the impulse response
+// h_i[m] = (1/L) sum_k psih_i[k] e^{2 pi i k m / L} (and its derivative
+// kernel), truncated where |h| < 1e-9 max|h|.  Blocks of B = F - 2H outputs
+// take F padded samples, one forward F-point FFT shared by a group of scales,
+// a product with each scale's precomputed kernel spectrum (including 1/F)
+// and an inverse FFT per channel; outputs t in [H, H+B) of the circular
+// convolution are the linear ones.  Padded indices wrap modulo L exactly as
+// the reference's circular convolution does.
+struct OsGroup {
+  int i0, ns, H, B, rank, pad0, pad1, pad2;   // scales [i0, i0+ns), coeff rank
+};
+
+template <int F>
+struct OsShape {
+  static constexpr int SR = (4096 / F) > 8 ? 8 : (4096 / F);  // scales per batch
+  static constexpr int NT = 256;
+};
+
+template <int F>
+__global__ __launch_bounds__(256) void k_cwt_os(CwtArgs a, const OsGroup* __restrict__ groups,
+                                                 const int2* __restrict__ tasks,
+                                                 const float2* __restrict__ coef,
+                                                 const float2* __restrict__ tw,
+                                                 const float* __restrict__ x, int64_t x_stride,
+                                                 int pad) {
+  constexpr int SR = OsShape<F>::SR;
+  constexpr int NT = OsShape<F>::NT;
+  __shared__ float2 sS[Lds<F>::elems(1)];
+  __shared__ float2 sA[Lds<F>::elems(SR)];
+  __shared__ float2 sD[Lds<F>::elems(SR)];
+  const int2 t = tasks[blockIdx.x];
+  const OsGroup g = groups[t.x];
+  const int b = blockIdx.y;
+  const int64_t L = a.L;
+  const int64_t j_base = (int64_t)t.y * g.B;              // first unpadded output
+  const int64_t start = (int64_t)a.n1 + j_base - g.H;     // padded index of seg[0]
+  const float* xb = x + (int64_t)b * x_stride;
+  for (int e = threadIdx.x; e < F; e += NT) {
+    int64_t n = (start + e) % L;
+    if (n < 0) n += L;
+    const int64_t src = pad_src(n, a.N, a.n1, pad);
+    sS[Lds<F>::idx(0, e)] = make_float2(src >= 0 ? xb[src] : 0.f, 0.f);
+  }
+  {
+    float2* const bufs[1] = {sS};
+    lds_fft<F, 1, -1, NT, 1>(bufs, tw);
+  }
+  for (int s0 = 0; s0 < g.ns; s0 += SR) {
+    for (int e = threadIdx.x; e < SR * F; e += NT) {
+      const int r = e / F, k = e - r * F;
+      float2 A = make_float2(0.f, 0.f), Dv = A;
+      if (s0 + r < g.ns) {
+        const float2 S = sS[Lds<F>::idx(0, k)];
+        const float2* c = coef + (int64_t)(g.rank + s0 + r) * 2 * F;
+        A = cmul(S, c[k]);
+        Dv = cmul(S, c[F + k]);
+      }
+      sA[Lds<F>::idx(r, k)] = A;
+      sD[Lds<F>::idx(r, k)] = Dv;
+    }
+    float2* const bufs[2] = {sA, sD};
+    lds_fft<F, SR, +1, NT, 2>(bufs, tw);
+    for (int e = threadIdx.x; e < SR * g.B; e += NT) {
+      const int r = e / g.B, tt = e - r * g.B;
+      if (s0 + r >= g.ns) continue;
+      const int64_t j = j_base + tt;
+      if (j >= a.N) continue;
+      const int pos = g.H + tt;
+      emit(a, b, g.i0 + s0 + r, (int64_t)a.n1 + j, sA[Lds<F>::idx(r, pos)],
+           sD[Lds<F>::idx(r, pos)]);
+    }
+    __syncthreads();
+  }
+}
+
+// Kernel spectra of the overlap-save path: row (rank) of the impulse
+// responses h (Wx of a unit impulse) and hd (its dWx), centred at column
+// hc, placed circularly for |m| <= H, forward F-point FFT, times 1/F.
+// task = {group, r}.
+template <int F>
+__global__ __launch_bounds__(256) void k_os_prep(const OsGroup* __restrict__ groups,
+                                                  const int2* __restrict__ tasks,
+                                                  const float2* __restrict__ hW,
+                                                  const float2* __restrict__ hD, int64_t hN,
+                                                  int hc, int rank0, const float2* __restrict__ tw,
+                                                  float2* __restrict__ coef) {
+  __shared__ float2 sA[Lds<F>::elems(1)];
+  __shared__ float2 sD[Lds<F>::elems(1)];
+  const int2 t = tasks[blockIdx.x];
+  const OsGroup g = groups[t.x];
+  const int rank = g.rank + t.y;
+  const int64_t row = (int64_t)(rank - rank0) * hN;
+  for (int e = threadIdx.x; e < F; e += 256) {
+    const int m = (e <= F / 2) ? e : e - F;
+    float2 A = make_float2(0.f, 0.f), Dv = A;
+    if (m <= g.H && -m <= g.H) {
+      A = hW[row + hc + m];
+      Dv = hD[row + hc + m];
+    }
+    sA[Lds<F>::idx(0, e)] = A;
+    sD[Lds<F>::idx(0, e)] = Dv;
+  }
+  float2* const bufs[2] = {sA, sD};
+  lds_fft<F, 1, -1, 256, 2>(bufs, tw);
+  const float invF = 1.0f / (float)F;
+  float2* c = coef + (int64_t)rank * 2 * F;
+  for (int k = threadIdx.x; k < F; k += 256) {
+    const float2 A = sA[Lds<F>::idx(0, k)], Dv = sD[Lds<F>::idx(0, k)];
+    c[k] = make_float2(A.x * invF, A.y * invF);
+    c[F + k] = make_float2(Dv.x * invF, Dv.y * invF);
+  }
+}
+
 // --------------------------------------------------- forward FFT of x_pad
 struct FwdArgs {
   const float* x;
@@ -382,6 +498,46 @@ int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
   return SSQK_OK;
 }
 
+#define SSQK_OS_CASES(MACRO) MACRO(256) MACRO(512) MACRO(1024) MACRO(2048) MACRO(4096)
+
+int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup* groups,
+              const int2* tasks, const float2* coef, const float* x, int64_t x_stride,
+              int pad) {
+  SSQK_GET_TW(tw, F);
+  switch (F) {
+#define C(m)                                                                        \
+  case m:                                                                           \
+    hipLaunchKernelGGL(k_cwt_os<m>, grid, dim3(256), 0, st, a, groups, tasks, coef, \
+                       tw, x, x_stride, pad);                                       \
+    break;
+    SSQK_OS_CASES(C)
+#undef C
+    default:
+      ssqk_host::set_error("overlap-save size %d unsupported", F);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  return SSQK_OK;
+}
+
+int launch_os_prep(int F, dim3 grid, hipStream_t st, const OsGroup* groups, const int2* tasks,
+                   const float2* hW, const float2* hD, int64_t hN, int hc, int rank0,
+                   float2* coef) {
+  SSQK_GET_TW(tw, F);
+  switch (F) {
+#define C(m)                                                                        \
+  case m:                                                                           \
+    hipLaunchKernelGGL(k_os_prep<m>, grid, dim3(256), 0, st, groups, tasks, hW, hD, \
+                       hN, hc, rank0, tw, coef);                                    \
+    break;
+    SSQK_OS_CASES(C)
+#undef C
+    default:
+      ssqk_host::set_error("overlap-save size %d unsupported", F);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  return SSQK_OK;
+}
+
 int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, float2* scr) {
   const int L = f.L;
   if (L <= kSmallFwd) {
@@ -468,6 +624,53 @@ int ilog2(int64_t v) {
   return r;
 }
 
+
+// ----------------------------------------------------- host FFT (plan time)
+void host_fft(std::vector<double>& re, std::vector<double>& im, int sign) {
+  const size_t n = re.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const double ang = sign * 2 * M_PI / (double)len;
+    for (size_t i = 0; i < n; i += len) {
+      for (size_t k = 0; k < len / 2; ++k) {
+        const double c = std::cos(ang * k), sn = std::sin(ang * k);
+        const size_t u = i + k, v = i + k + len / 2;
+        const double tr = re[v] * c - im[v] * sn, ti = re[v] * sn + im[v] * c;
+        re[v] = re[u] - tr; im[v] = im[u] - ti;
+        re[u] += tr; im[u] += ti;
+      }
+    }
+  }
+}
+
+// Half support, in samples per unit scale, of the time-domain wavelet: the
+// impulse response at a reference scale well inside the band, truncated
+// where |h| < eps max|h|.  The discrete kernel of any scale whose band is
+// not clipped by the Nyquist bin has support H(s) ~ s * T1.
+double time_support_per_scale(const WavDev& p, double eps) {
+  const int Lr = 1 << 16;
+  const double s_ref = 16.0;
+  std::vector<double> re(Lr, 0.0), im(Lr, 0.0);
+  for (int k = 0; k <= Lr / 2; ++k) re[k] = psih_host(p, s_ref * 2 * M_PI * k / Lr);
+  re[Lr / 2] *= 0.5;
+  host_fft(re, im, +1);
+  double mx = 0;
+  for (int m = 0; m < Lr; ++m) mx = std::max(mx, std::hypot(re[m], im[m]));
+  int H = 0;
+  for (int m = 0; m < Lr; ++m) {
+    if (std::hypot(re[m], im[m]) > eps * mx) {
+      const int d = m <= Lr / 2 ? m : Lr - m;
+      H = std::max(H, d);
+    }
+  }
+  return (double)H / s_ref;
+}
+
 }  // namespace
 
 // ================================================================== plan
@@ -496,13 +699,296 @@ struct ssqk_cwt_plan_s {
   };
   std::vector<Chunk> chunks;
   int fL1 = 0, fL2 = 0;       // forward four-step split (L > kSmallFwd)
+  // overlap-save path: per FFT size, groups of consecutive scales
+  struct OsCls {
+    int F = 0;
+    int ntasks = 0;
+    OsGroup* d_groups = nullptr;
+    int2* d_tasks = nullptr;
+  };
+  std::vector<OsCls> os;
+  float2* d_coef = nullptr;   // [rank][2][F]
+  int n_os = 0;
+  int path_count[3] = {0, 0, 0};   // band, overlap-save, four-step
 };
 
-static void split_four_step(int64_t L, int* L1, int* L2) {
+namespace {
+
+void split_four_step(int64_t L, int* L1, int* L2) {
   const int lg = ilog2(L);
   *L2 = 1 << ((lg + 1) / 2);
   *L1 = (int)(L / *L2);
 }
+
+int64_t xh_stride(const ssqk_cwt_plan_s* p) { return ((p->L / 2 + 1 + 63) / 64) * 64; }
+
+void ws_layout(const ssqk_cwt_plan_s* p, int64_t batch, size_t* off_fwd, size_t* off_full,
+               size_t* total) {
+  size_t o = sizeof(float2) * (size_t)(xh_stride(p) * batch);
+  o = (o + 255) & ~size_t(255);
+  *off_fwd = o;
+  if (p->L > kSmallFwd) o += sizeof(float2) * (size_t)(p->L * batch);
+  o = (o + 255) & ~size_t(255);
+  *off_full = o;
+  if (!p->full.empty()) o += sizeof(float2) * (size_t)(2 * p->L * batch * p->slots);
+  *total = o;
+}
+
+template <typename T>
+hipError_t upload(T** d, const std::vector<T>& h) {
+  hipError_t e = hipMalloc((void**)d, sizeof(T) * std::max<size_t>(h.size(), 1));
+  if (e == hipSuccess && !h.empty())
+    e = hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+  return e;
+}
+
+struct PlanOpts {
+  bool allow_os = true;
+};
+
+int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, int pad_type,
+                     const WavDev& wav, double center, const double* scales, int64_t na,
+                     double dt, double band_rel, PlanOpts opt);
+int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void* dWx,
+              float* w, uint16_t* bins, const ssqk_bin_params* bp, double gamma,
+              int gamma_strict, void* workspace, hipStream_t st);
+
+// Kernel spectra of the overlap-save scales: an internal plan (band and
+// four-step paths only) computes Wx and dWx of a unit impulse centred in a
+// window of 2 Hmax + 1 samples, i.e. h_i[m] and hd_i[m] for |m| <= Hmax;
+// k_os_prep turns each row into its F-point spectrum.
+int build_os_coefficients(ssqk_cwt_plan_s* p, const std::vector<int>& os_scales,
+                          const std::vector<double>& scales_h, int Hmax,
+                          const std::vector<std::vector<int2>>& prep_tasks,
+                          const std::vector<OsGroup>& all_groups) {
+  const int64_t hN = 2 * (int64_t)Hmax + 1;
+  const int64_t hn1 = p->L / 2 - Hmax;
+  std::vector<double> s_os(os_scales.size());
+  for (size_t r = 0; r < os_scales.size(); ++r) s_os[r] = scales_h[os_scales[r]];
+  ssqk_cwt_plan_s* imp = nullptr;
+  PlanOpts o;
+  o.allow_os = false;
+  int rc = plan_create_impl(&imp, hN, p->L, hn1, SSQK_PAD_ZERO, p->wav, 1.0, s_os.data(),
+                            (int64_t)s_os.size(), p->dt, 0.0, o);
+  if (rc) return rc;
+  size_t off_fwd, off_full, wsz;
+  ws_layout(imp, 1, &off_fwd, &off_full, &wsz);
+  float* d_x = nullptr;
+  float2 *d_hW = nullptr, *d_hD = nullptr;
+  void* d_ws = nullptr;
+  hipError_t e = hipMalloc(&d_x, sizeof(float) * hN);
+  if (e == hipSuccess) e = hipMemset(d_x, 0, sizeof(float) * hN);
+  const float one = 1.0f;
+  if (e == hipSuccess) e = hipMemcpy(d_x + Hmax, &one, sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&d_hW, sizeof(float2) * hN * s_os.size());
+  if (e == hipSuccess) e = hipMalloc(&d_hD, sizeof(float2) * hN * s_os.size());
+  if (e == hipSuccess) e = hipMalloc(&d_ws, wsz);
+  if (e == hipSuccess) {
+    rc = exec_impl(imp, d_x, 1, d_hW, d_hD, nullptr, nullptr, nullptr, 0.0, 1, d_ws, nullptr);
+  }
+  OsGroup* d_groups_all = nullptr;
+  if (e == hipSuccess && rc == 0) e = upload(&d_groups_all, all_groups);
+  for (size_t c = 0; c < p->os.size() && e == hipSuccess && rc == 0; ++c) {
+    int2* d_pt = nullptr;
+    e = upload(&d_pt, prep_tasks[c]);
+    if (e == hipSuccess)
+      rc = launch_os_prep(p->os[c].F, dim3((unsigned)prep_tasks[c].size()), nullptr,
+                          d_groups_all, d_pt, d_hW, d_hD, hN, Hmax, 0, p->d_coef);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d_pt);
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(d_groups_all);
+  (void)hipFree(d_x);
+  (void)hipFree(d_hW);
+  (void)hipFree(d_hD);
+  (void)hipFree(d_ws);
+  ssqk_cwt_plan_destroy(imp);
+  if (rc) return rc;
+  if (e != hipSuccess) return ssqk_host::check_hip(e, "overlap-save preparation");
+  return SSQK_OK;
+}
+
+int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, int pad_type,
+                     const WavDev& wav, double center, const double* scales, int64_t na,
+                     double dt, double band_rel, PlanOpts opt) {
+  ssqk_cwt_plan_s* p = new (std::nothrow) ssqk_cwt_plan_s();
+  if (!p) {
+    ssqk_host::set_error("out of host memory");
+    return SSQK_ERR_ARG;
+  }
+  p->N = N; p->L = L; p->n1 = n1; p->pad = pad_type; p->na = na; p->dt = dt;
+  p->wav = wav;
+  double wlo, whi;
+  band_limits(p->wav, center, band_rel > 0 ? band_rel : 1e-10, &wlo, &whi);
+
+  const int64_t half = L / 2;
+  const double dxi = 2 * M_PI / (double)L;
+  const double ratio = (double)L / (double)N;
+  const double T1 = opt.allow_os ? time_support_per_scale(p->wav, 1e-9) : 0.0;
+  std::vector<double> scales_h(scales, scales + na);
+  std::vector<int2> band(na);
+  std::vector<std::vector<int4>> cls_tasks(14);
+  std::vector<int> os_F(na, 0), os_H(na, 0);
+  for (int64_t i = 0; i < na; ++i) {
+    const double s = scales[i];
+    if (!(s > 0)) {
+      delete p;
+      ssqk_host::set_error("scales must be positive (scale %lld = %g)", (long long)i, s);
+      return SSQK_ERR_ARG;
+    }
+    const double ka_f = std::ceil(wlo / (s * dxi)), kb_f = std::floor(whi / (s * dxi));
+    int64_t ka = (int64_t)std::max(ka_f, 0.0);
+    int64_t kb = (int64_t)std::min(kb_f, (double)half);
+    if (ka > kb) { ka = 0; kb = -1; }        // empty band: row of zeros
+    band[i] = make_int2((int)ka, (int)kb);
+    const int64_t K = std::max<int64_t>(kb - ka + 1, 1);
+    int64_t M = std::max<int64_t>(int64_t(1) << ilog2(K), 2);
+    M = std::min<int64_t>(M, L);
+    // cost per needed output sample, in FFT "bits" with rough overheads
+    double cost = M <= kP ? (std::log2((double)M) + 3.0) * ratio
+                          : (std::log2((double)L) + 8.0) * ratio;
+    int bestF = 0, bestH = 0;
+    // overlap-save needs a band that the Nyquist bin does not clip (else the
+    // kernel's time tail is long) and a block comfortably longer than 2H
+    if (opt.allow_os && kb_f < (double)half - 1 && K > 1) {
+      const int H = (int)std::ceil(T1 * s * 1.02) + 2;
+      for (int F = 256; F <= 4096; F *= 2) {
+        if (F < 2 * H + 64 || 2 * (int64_t)H + 1 > N) continue;
+        const double c = (std::log2((double)F) + 3.0) * F / (double)(F - 2 * H) +
+                         std::log2((double)F) / 8.0;
+        if (c < cost) { cost = c; bestF = F; bestH = H; }
+      }
+    }
+    if (bestF) {
+      os_F[i] = bestF;
+      os_H[i] = bestH;
+      continue;
+    }
+    if (M <= kP) {
+      const int lM = ilog2(M);
+      const int64_t D = L / M;
+      const int rows = band_rows((int)M);
+      for (int64_t a0 = 0; a0 < D; a0 += rows)
+        cls_tasks[lM].push_back(make_int4((int)i, (int)a0, (int)ka, ilog2(D)));
+      p->path_count[0]++;
+    } else {
+      p->full.push_back((int)i);
+      p->path_count[2]++;
+    }
+  }
+  hipError_t e = hipSuccess;
+  e = hipMalloc(&p->d_scales, sizeof(double) * na);
+  if (e == hipSuccess) e = hipMemcpy(p->d_scales, scales, sizeof(double) * na, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = upload(&p->d_band, band);
+  for (int lM = 0; lM < 14 && e == hipSuccess; ++lM) {
+    if (cls_tasks[lM].empty()) continue;
+    ssqk_cwt_plan_s::Cls c;
+    c.M = 1 << lM;
+    c.ntasks = (int)cls_tasks[lM].size();
+    e = upload(&c.d_tasks, cls_tasks[lM]);
+    p->classes.push_back(c);
+  }
+  if (!p->full.empty() && e == hipSuccess) {
+    split_four_step(L, &p->L1, &p->L2);
+    const int64_t per_slot = L * 2 * (int64_t)sizeof(float2);
+    p->slots = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)p->full.size(),
+                                                            (64ll << 20) / per_slot));
+    const int GA = kP / p->L2, GB = kP / p->L1;
+    for (size_t c0 = 0; c0 < p->full.size() && e == hipSuccess; c0 += p->slots) {
+      std::vector<int4> A, B;
+      for (int sl = 0; sl < p->slots && c0 + sl < p->full.size(); ++sl) {
+        const int i = p->full[c0 + sl];
+        for (int k1 = 0; k1 < p->L1; k1 += GA) A.push_back(make_int4(i, k1, sl, 0));
+        for (int n2 = 0; n2 < p->L2; n2 += GB) B.push_back(make_int4(i, n2, sl, 0));
+      }
+      ssqk_cwt_plan_s::Chunk ch;
+      ch.nA = (int)A.size();
+      ch.nB = (int)B.size();
+      e = upload(&ch.d_A, A);
+      if (e == hipSuccess) e = upload(&ch.d_B, B);
+      p->chunks.push_back(ch);
+    }
+  }
+  if (L > kSmallFwd) split_four_step(L, &p->fL1, &p->fL2);
+
+  // overlap-save groups: runs of consecutive scales sharing F, up to 8
+  std::vector<int> os_scales;
+  std::vector<OsGroup> all_groups;
+  std::vector<std::vector<int2>> prep_tasks;
+  int Hmax = 0;
+  if (e == hipSuccess) {
+    std::vector<std::vector<OsGroup>> grp_by_F;
+    std::vector<int> Fs;
+    for (int64_t i = 0; i < na;) {
+      if (!os_F[i]) { ++i; continue; }
+      const int F = os_F[i];
+      OsGroup g{};
+      g.i0 = (int)i;
+      g.ns = 0;
+      g.H = 0;
+      g.rank = (int)os_scales.size();
+      while (i < na && os_F[i] == F && g.ns < 8) {
+        g.H = std::max(g.H, os_H[i]);
+        os_scales.push_back((int)i);
+        ++g.ns;
+        ++i;
+      }
+      g.B = F - 2 * g.H;
+      Hmax = std::max(Hmax, g.H);
+      size_t c = std::find(Fs.begin(), Fs.end(), F) - Fs.begin();
+      if (c == Fs.size()) { Fs.push_back(F); grp_by_F.emplace_back(); }
+      grp_by_F[c].push_back(g);
+      p->path_count[1] += g.ns;
+    }
+    p->n_os = (int)os_scales.size();
+    for (size_t c = 0; c < Fs.size() && e == hipSuccess; ++c) {
+      ssqk_cwt_plan_s::OsCls oc;
+      oc.F = Fs[c];
+      std::vector<int2> tasks, pt;
+      for (size_t gi = 0; gi < grp_by_F[c].size(); ++gi) {
+        const OsGroup& g = grp_by_F[c][gi];
+        const int64_t nblk = (N + g.B - 1) / g.B;
+        for (int64_t bk = 0; bk < nblk; ++bk) tasks.push_back(make_int2((int)gi, (int)bk));
+        for (int r = 0; r < g.ns; ++r)
+          pt.push_back(make_int2((int)(all_groups.size() + gi), r));
+      }
+      oc.ntasks = (int)tasks.size();
+      e = upload(&oc.d_groups, grp_by_F[c]);
+      if (e == hipSuccess) e = upload(&oc.d_tasks, tasks);
+      all_groups.insert(all_groups.end(), grp_by_F[c].begin(), grp_by_F[c].end());
+      prep_tasks.push_back(pt);
+      p->os.push_back(oc);
+    }
+    if (e == hipSuccess && p->n_os) {
+      size_t coef_elems = 0;
+      for (const auto& oc : p->os) {
+        (void)oc;
+      }
+      // coefficient rows are indexed by rank with stride 2F of their class;
+      // allocate for the largest F to keep one table
+      int Fmax = 0;
+      for (const auto& oc : p->os) Fmax = std::max(Fmax, oc.F);
+      coef_elems = (size_t)p->n_os * 2 * Fmax;
+      e = hipMalloc(&p->d_coef, sizeof(float2) * coef_elems);
+    }
+  }
+  if (e != hipSuccess) {
+    ssqk_cwt_plan_destroy(p);
+    return ssqk_host::check_hip(e, "plan upload");
+  }
+  if (p->n_os) {
+    const int rc = build_os_coefficients(p, os_scales, scales_h, Hmax, prep_tasks, all_groups);
+    if (rc) {
+      ssqk_cwt_plan_destroy(p);
+      return rc;
+    }
+  }
+  *out = p;
+  return SSQK_OK;
+}
+
+}  // namespace
 
 extern "C" int ssqk_cwt_plan_create(ssqk_cwt_plan* out, int64_t N, int64_t L, int64_t n1,
                                     int pad_type, int wavelet_kind,
@@ -522,112 +1008,38 @@ extern "C" int ssqk_cwt_plan_create(ssqk_cwt_plan* out, int64_t N, int64_t L, in
                          (long long)L, (long long)n1, (long long)na);
     return SSQK_ERR_ARG;
   }
-  if (wavelet_kind != SSQK_WAVELET_GMW && wavelet_kind != SSQK_WAVELET_MORLET) {
-    ssqk_host::set_error("unknown wavelet kind %d", wavelet_kind);
-    return SSQK_ERR_UNSUPPORTED;
-  }
-  ssqk_cwt_plan_s* p = new (std::nothrow) ssqk_cwt_plan_s();
-  if (!p) {
-    ssqk_host::set_error("out of host memory");
+  if (pad_type < SSQK_PAD_REFLECT || pad_type > SSQK_PAD_WRAP) {
+    ssqk_host::set_error("unknown pad type %d", pad_type);
     return SSQK_ERR_ARG;
   }
-  p->N = N; p->L = L; p->n1 = n1; p->pad = pad_type; p->na = na; p->dt = dt;
+  WavDev wav{};
   double center;
   if (wavelet_kind == SSQK_WAVELET_GMW) {
     const double gamma = wavelet_params[0], beta = wavelet_params[1];
     if (!(gamma > 0) || !(beta > 0)) {
-      delete p;
       ssqk_host::set_error("GMW needs gamma > 0 and beta > 0");
       return SSQK_ERR_ARG;
     }
     const double wc = std::pow(beta / gamma, 1.0 / gamma);
-    p->wav = WavDev{SSQK_WAVELET_GMW, gamma, beta, std::log(wc), std::pow(wc, gamma)};
+    wav = WavDev{SSQK_WAVELET_GMW, gamma, beta, std::log(wc), std::pow(wc, gamma)};
     center = wc;
-  } else {
+  } else if (wavelet_kind == SSQK_WAVELET_MORLET) {
     const double mu = wavelet_params[0];
     const double cs = std::pow(1 + std::exp(-mu * mu) - 2 * std::exp(-0.75 * mu * mu), -0.5);
     const double ks = std::exp(-0.5 * mu * mu);
-    p->wav = WavDev{SSQK_WAVELET_MORLET, mu, ks, -0.5,
-                    std::sqrt(2.0) * cs * std::pow(M_PI, 0.25)};
+    wav = WavDev{SSQK_WAVELET_MORLET, mu, ks, -0.5, std::sqrt(2.0) * cs * std::pow(M_PI, 0.25)};
     center = mu > 0 ? mu : 1.0;
+  } else {
+    ssqk_host::set_error("unknown wavelet kind %d", wavelet_kind);
+    return SSQK_ERR_UNSUPPORTED;
   }
-  double wlo, whi;
-  band_limits(p->wav, center, band_rel > 0 ? band_rel : 1e-10, &wlo, &whi);
-
-  const int64_t half = L / 2;
-  const double dxi = 2 * M_PI / (double)L;
-  std::vector<int2> band(na);
-  std::vector<std::vector<int4>> cls_tasks(14);
-  for (int64_t i = 0; i < na; ++i) {
-    const double s = scales[i];
-    if (!(s > 0)) {
-      delete p;
-      ssqk_host::set_error("scales must be positive (scale %lld = %g)", (long long)i, s);
-      return SSQK_ERR_ARG;
-    }
-    int64_t ka = (int64_t)std::ceil(wlo / (s * dxi));
-    int64_t kb = (int64_t)std::floor(whi / (s * dxi));
-    ka = std::max<int64_t>(ka, 0);
-    kb = std::min<int64_t>(kb, half);
-    if (ka > kb) { ka = 0; kb = -1; }        // empty band: row of zeros
-    band[i] = make_int2((int)ka, (int)kb);
-    const int64_t K = std::max<int64_t>(kb - ka + 1, 1);
-    int64_t M = std::max<int64_t>(int64_t(1) << ilog2(K), 2);
-    M = std::min<int64_t>(M, L);
-    if (M <= kP) {
-      const int lM = ilog2(M);
-      const int64_t D = L / M;
-      const int rows = band_rows((int)M);
-      for (int64_t a0 = 0; a0 < D; a0 += rows)
-        cls_tasks[lM].push_back(make_int4((int)i, (int)a0, (int)ka, ilog2(D)));
-    } else {
-      p->full.push_back((int)i);
-    }
-  }
-  hipError_t e = hipSuccess;
-  e = hipMalloc(&p->d_scales, sizeof(double) * na);
-  if (e == hipSuccess) e = hipMemcpy(p->d_scales, scales, sizeof(double) * na, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc(&p->d_band, sizeof(int2) * na);
-  if (e == hipSuccess) e = hipMemcpy(p->d_band, band.data(), sizeof(int2) * na, hipMemcpyHostToDevice);
-  for (int lM = 0; lM < 14 && e == hipSuccess; ++lM) {
-    if (cls_tasks[lM].empty()) continue;
-    ssqk_cwt_plan_s::Cls c;
-    c.M = 1 << lM;
-    c.ntasks = (int)cls_tasks[lM].size();
-    e = hipMalloc(&c.d_tasks, sizeof(int4) * c.ntasks);
-    if (e == hipSuccess)
-      e = hipMemcpy(c.d_tasks, cls_tasks[lM].data(), sizeof(int4) * c.ntasks, hipMemcpyHostToDevice);
-    p->classes.push_back(c);
-  }
-  if (!p->full.empty() && e == hipSuccess) {
-    split_four_step(L, &p->L1, &p->L2);
-    // scratch budget ~64 MiB per signal of the batch
-    const int64_t per_slot = L * 2 * (int64_t)sizeof(float2);
-    p->slots = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)p->full.size(),
-                                                            (64ll << 20) / per_slot));
-    const int GA = kP / p->L2, GB = kP / p->L1;
-    for (size_t c0 = 0; c0 < p->full.size() && e == hipSuccess; c0 += p->slots) {
-      std::vector<int4> A, B;
-      for (int sl = 0; sl < p->slots && c0 + sl < p->full.size(); ++sl) {
-        const int i = p->full[c0 + sl];
-        for (int k1 = 0; k1 < p->L1; k1 += GA) A.push_back(make_int4(i, k1, sl, 0));
-        for (int n2 = 0; n2 < p->L2; n2 += GB) B.push_back(make_int4(i, n2, sl, 0));
-      }
-      ssqk_cwt_plan_s::Chunk ch;
-      ch.nA = (int)A.size();
-      ch.nB = (int)B.size();
-      e = hipMalloc(&ch.d_A, sizeof(int4) * A.size());
-      if (e == hipSuccess) e = hipMemcpy(ch.d_A, A.data(), sizeof(int4) * A.size(), hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = hipMalloc(&ch.d_B, sizeof(int4) * B.size());
-      if (e == hipSuccess) e = hipMemcpy(ch.d_B, B.data(), sizeof(int4) * B.size(), hipMemcpyHostToDevice);
-      p->chunks.push_back(ch);
-    }
-  }
-  if (L > kSmallFwd) split_four_step(L, &p->fL1, &p->fL2);
-  if (e != hipSuccess) {
-    ssqk_cwt_plan_destroy(p);
-    return ssqk_host::check_hip(e, "plan upload");
-  }
+  ssqk_cwt_plan_s* p = nullptr;
+  const char* env = std::getenv("SSQK_DISABLE_OVERLAP_SAVE");
+  PlanOpts opt;
+  opt.allow_os = !(env && env[0] == '1');
+  const int rc = plan_create_impl(&p, N, L, n1, pad_type, wav, center, scales, na, dt,
+                                  band_rel, opt);
+  if (rc) return rc;
   *out = p;
   return SSQK_OK;
 }
@@ -641,24 +1053,22 @@ extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
     (void)hipFree(c.d_A);
     (void)hipFree(c.d_B);
   }
+  for (auto& c : p->os) {
+    (void)hipFree(c.d_groups);
+    (void)hipFree(c.d_tasks);
+  }
+  (void)hipFree(p->d_coef);
   delete p;
   return SSQK_OK;
 }
 
-static int64_t xh_stride(const ssqk_cwt_plan_s* p) {
-  return ((p->L / 2 + 1 + 63) / 64) * 64;
-}
-
-static void ws_layout(const ssqk_cwt_plan_s* p, int64_t batch, size_t* off_fwd,
-                      size_t* off_full, size_t* total) {
-  size_t o = sizeof(float2) * (size_t)(xh_stride(p) * batch);
-  o = (o + 255) & ~size_t(255);
-  *off_fwd = o;
-  if (p->L > kSmallFwd) o += sizeof(float2) * (size_t)(p->L * batch);
-  o = (o + 255) & ~size_t(255);
-  *off_full = o;
-  if (!p->full.empty()) o += sizeof(float2) * (size_t)(2 * p->L * batch * p->slots);
-  *total = o;
+extern "C" int ssqk_cwt_plan_paths(ssqk_cwt_plan p, int64_t* counts) {
+  if (!p || !counts) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_plan_paths");
+    return SSQK_ERR_ARG;
+  }
+  for (int k = 0; k < 3; ++k) counts[k] = p->path_count[k];
+  return SSQK_OK;
 }
 
 extern "C" int ssqk_cwt_workspace_size(ssqk_cwt_plan p, int64_t batch, size_t* bytes) {
@@ -671,39 +1081,14 @@ extern "C" int ssqk_cwt_workspace_size(ssqk_cwt_plan p, int64_t batch, size_t* b
   return SSQK_OK;
 }
 
-extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, void* Wx,
-                             void* dWx, float* w, uint16_t* bins, const ssqk_bin_params* bp,
-                             double gamma, int gamma_strict, void* workspace, void* stream) {
-  if (!p || !x || !Wx || !workspace || batch < 1 || batch > 65535) {
-    ssqk_host::set_error("bad arguments to ssqk_cwt_exec");
-    return SSQK_ERR_ARG;
-  }
-  if (bins && !bp) {
-    ssqk_host::set_error("bins requested without bin parameters");
-    return SSQK_ERR_ARG;
-  }
-  if (bins && bp && (bp->omax < 0 || bp->omax >= kNoBin)) {
-    ssqk_host::set_error("omax=%lld out of range for uint16 bins", (long long)bp->omax);
-    return SSQK_ERR_ARG;
-  }
-  hipStream_t st = (hipStream_t)stream;
+namespace {
+int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void* dWx,
+              float* w, uint16_t* bins, const ssqk_bin_params* bp, double gamma,
+              int gamma_strict, void* workspace, hipStream_t st) {
   size_t off_fwd, off_full, total;
   ws_layout(p, batch, &off_fwd, &off_full, &total);
   char* ws = (char*)workspace;
   float2* xh = (float2*)ws;
-
-  FwdArgs f;
-  f.x = x; f.N = p->N; f.n1 = p->n1; f.L = (int)p->L; f.pad = p->pad;
-  f.xh = xh; f.xh_stride = xh_stride(p);
-  f.ptab = ssqk_host::phase_table((int)p->L);
-  f.pS = phase_tab_split(p->L);
-  if (!f.ptab) {
-    ssqk_host::set_error("could not create the phase table");
-    return SSQK_ERR_HIP;
-  }
-  int rc = launch_fwd(f, (int)batch, st, p->fL1, p->fL2, (float2*)(ws + off_fwd));
-  if (rc) return rc;
-  SSQK_CHECK(hipGetLastError());
 
   CwtArgs a;
   a.xh = xh; a.xh_stride = xh_stride(p);
@@ -717,8 +1102,29 @@ extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, voi
   a.wav = p->wav;
   ssqk_bin_params zero{};
   a.bp = make_bin_dev(bp ? bp : &zero);
-  a.ptab = f.ptab;
-  a.pS = f.pS;
+  a.ptab = ssqk_host::phase_table((int)p->L);
+  a.pS = phase_tab_split(p->L);
+  if (!a.ptab) {
+    ssqk_host::set_error("could not create the phase table");
+    return SSQK_ERR_HIP;
+  }
+  int rc;
+  if (!p->classes.empty() || !p->full.empty()) {
+    FwdArgs f;
+    f.x = x; f.N = p->N; f.n1 = p->n1; f.L = (int)p->L; f.pad = p->pad;
+    f.xh = xh; f.xh_stride = xh_stride(p);
+    f.ptab = a.ptab;
+    f.pS = a.pS;
+    rc = launch_fwd(f, (int)batch, st, p->fL1, p->fL2, (float2*)(ws + off_fwd));
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+  }
+  for (const auto& oc : p->os) {
+    rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), st, a, oc.d_groups, oc.d_tasks,
+                   p->d_coef, x, p->N, p->pad);
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+  }
   for (const auto& c : p->classes) {
     rc = launch_band(c.M, dim3(c.ntasks, (unsigned)batch), st, a, c.d_tasks);
     if (rc) return rc;
@@ -737,4 +1143,24 @@ extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, voi
     SSQK_CHECK(hipGetLastError());
   }
   return SSQK_OK;
+}
+}  // namespace
+
+extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, void* Wx,
+                             void* dWx, float* w, uint16_t* bins, const ssqk_bin_params* bp,
+                             double gamma, int gamma_strict, void* workspace, void* stream) {
+  if (!p || !x || !Wx || !workspace || batch < 1 || batch > 65535) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_exec");
+    return SSQK_ERR_ARG;
+  }
+  if (bins && !bp) {
+    ssqk_host::set_error("bins requested without bin parameters");
+    return SSQK_ERR_ARG;
+  }
+  if (bins && bp && (bp->omax < 0 || bp->omax >= kNoBin)) {
+    ssqk_host::set_error("omax=%lld out of range for uint16 bins", (long long)bp->omax);
+    return SSQK_ERR_ARG;
+  }
+  return exec_impl(p, x, batch, Wx, dWx, w, bins, bp, gamma, gamma_strict, workspace,
+                   (hipStream_t)stream);
 }
