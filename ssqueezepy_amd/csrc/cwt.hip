@@ -279,6 +279,7 @@ template <int F>
 __global__ __launch_bounds__(256) void k_cwt_os(CwtArgs a, const OsGroup* __restrict__ groups,
                                                  const int2* __restrict__ tasks,
                                                  const float2* __restrict__ coef,
+                                                 int coef_stride,
                                                  const float2* __restrict__ tw,
                                                  const float* __restrict__ x, int64_t x_stride,
                                                  int pad) {
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(256) void k_cwt_os(CwtArgs a, const OsGroup* __rest
       float2 A = make_float2(0.f, 0.f), Dv = A;
       if (s0 + r < g.ns) {
         const float2 S = sS[Lds<F>::idx(0, k)];
-        const float2* c = coef + (int64_t)(g.rank + s0 + r) * 2 * F;
+        const float2* c = coef + (int64_t)(g.rank + s0 + r) * coef_stride;
         A = cmul(S, c[k]);
         Dv = cmul(S, c[F + k]);
       }
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(256) void k_os_prep(const OsGroup* __restrict__ gro
                                                   const float2* __restrict__ hW,
                                                   const float2* __restrict__ hD, int64_t hN,
                                                   int hc, int rank0, const float2* __restrict__ tw,
-                                                  float2* __restrict__ coef) {
+                                                  float2* __restrict__ coef, int coef_stride) {
   __shared__ float2 sA[Lds<F>::elems(1)];
   __shared__ float2 sD[Lds<F>::elems(1)];
   const int2 t = tasks[blockIdx.x];
@@ -362,7 +363,7 @@ __global__ __launch_bounds__(256) void k_os_prep(const OsGroup* __restrict__ gro
   float2* const bufs[2] = {sA, sD};
   lds_fft<F, 1, -1, 256, 2>(bufs, tw);
   const float invF = 1.0f / (float)F;
-  float2* c = coef + (int64_t)rank * 2 * F;
+  float2* c = coef + (int64_t)rank * coef_stride;
   for (int k = threadIdx.x; k < F; k += 256) {
     const float2 A = sA[Lds<F>::idx(0, k)], Dv = sD[Lds<F>::idx(0, k)];
     c[k] = make_float2(A.x * invF, A.y * invF);
@@ -501,14 +502,14 @@ int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
 #define SSQK_OS_CASES(MACRO) MACRO(256) MACRO(512) MACRO(1024) MACRO(2048) MACRO(4096)
 
 int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup* groups,
-              const int2* tasks, const float2* coef, const float* x, int64_t x_stride,
-              int pad) {
+              const int2* tasks, const float2* coef, int coef_stride, const float* x,
+              int64_t x_stride, int pad) {
   SSQK_GET_TW(tw, F);
   switch (F) {
 #define C(m)                                                                        \
   case m:                                                                           \
     hipLaunchKernelGGL(k_cwt_os<m>, grid, dim3(256), 0, st, a, groups, tasks, coef, \
-                       tw, x, x_stride, pad);                                       \
+                       coef_stride, tw, x, x_stride, pad);                          \
     break;
     SSQK_OS_CASES(C)
 #undef C
@@ -521,13 +522,13 @@ int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup*
 
 int launch_os_prep(int F, dim3 grid, hipStream_t st, const OsGroup* groups, const int2* tasks,
                    const float2* hW, const float2* hD, int64_t hN, int hc, int rank0,
-                   float2* coef) {
+                   float2* coef, int coef_stride) {
   SSQK_GET_TW(tw, F);
   switch (F) {
 #define C(m)                                                                        \
   case m:                                                                           \
     hipLaunchKernelGGL(k_os_prep<m>, grid, dim3(256), 0, st, groups, tasks, hW, hD, \
-                       hN, hc, rank0, tw, coef);                                    \
+                       hN, hc, rank0, tw, coef, coef_stride);                       \
     break;
     SSQK_OS_CASES(C)
 #undef C
@@ -707,8 +708,10 @@ struct ssqk_cwt_plan_s {
     int2* d_tasks = nullptr;
   };
   std::vector<OsCls> os;
-  float2* d_coef = nullptr;   // [rank][2][F]
+  float2* d_coef = nullptr;   // [rank][2][F], row stride coef_stride
+  int coef_stride = 0;
   int n_os = 0;
+  double center = 1.0;        // peak radian frequency of the wavelet
   int path_count[3] = {0, 0, 0};   // band, overlap-save, four-step
 };
 
@@ -768,7 +771,7 @@ int build_os_coefficients(ssqk_cwt_plan_s* p, const std::vector<int>& os_scales,
   ssqk_cwt_plan_s* imp = nullptr;
   PlanOpts o;
   o.allow_os = false;
-  int rc = plan_create_impl(&imp, hN, p->L, hn1, SSQK_PAD_ZERO, p->wav, 1.0, s_os.data(),
+  int rc = plan_create_impl(&imp, hN, p->L, hn1, SSQK_PAD_ZERO, p->wav, p->center, s_os.data(),
                             (int64_t)s_os.size(), p->dt, 0.0, o);
   if (rc) return rc;
   size_t off_fwd, off_full, wsz;
@@ -793,7 +796,8 @@ int build_os_coefficients(ssqk_cwt_plan_s* p, const std::vector<int>& os_scales,
     e = upload(&d_pt, prep_tasks[c]);
     if (e == hipSuccess)
       rc = launch_os_prep(p->os[c].F, dim3((unsigned)prep_tasks[c].size()), nullptr,
-                          d_groups_all, d_pt, d_hW, d_hD, hN, Hmax, 0, p->d_coef);
+                          d_groups_all, d_pt, d_hW, d_hD, hN, Hmax, 0, p->d_coef,
+                          p->coef_stride);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(d_pt);
   }
@@ -819,6 +823,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
   }
   p->N = N; p->L = L; p->n1 = n1; p->pad = pad_type; p->na = na; p->dt = dt;
   p->wav = wav;
+  p->center = center;
   double wlo, whi;
   band_limits(p->wav, center, band_rel > 0 ? band_rel : 1e-10, &wlo, &whi);
 
@@ -961,16 +966,11 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       p->os.push_back(oc);
     }
     if (e == hipSuccess && p->n_os) {
-      size_t coef_elems = 0;
-      for (const auto& oc : p->os) {
-        (void)oc;
-      }
-      // coefficient rows are indexed by rank with stride 2F of their class;
-      // allocate for the largest F to keep one table
+      // one table, rows indexed by rank with the stride of the largest F
       int Fmax = 0;
       for (const auto& oc : p->os) Fmax = std::max(Fmax, oc.F);
-      coef_elems = (size_t)p->n_os * 2 * Fmax;
-      e = hipMalloc(&p->d_coef, sizeof(float2) * coef_elems);
+      p->coef_stride = 2 * Fmax;
+      e = hipMalloc(&p->d_coef, sizeof(float2) * (size_t)p->n_os * p->coef_stride);
     }
   }
   if (e != hipSuccess) {
@@ -1121,7 +1121,7 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
   }
   for (const auto& oc : p->os) {
     rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), st, a, oc.d_groups, oc.d_tasks,
-                   p->d_coef, x, p->N, p->pad);
+                   p->d_coef, p->coef_stride, x, p->N, p->pad);
     if (rc) return rc;
     SSQK_CHECK(hipGetLastError());
   }
