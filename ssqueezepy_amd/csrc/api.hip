@@ -198,26 +198,25 @@ __global__ __launch_bounds__(kSqNT) void k_squeeze(const float2* __restrict__ Wx
   const uint16_t* bp_ = bins + (int64_t)b * in_stride + (live ? j : 0);
 
   for (int i0 = 0; i0 < na; i0 += kSqU) {
+    // unconditional (clamped) loads so the compiler issues all kSqU of them
+    // before the first use; invalid rows are masked afterwards
     float2 wv[kSqU];
     uint32_t kv[kSqU];
+    float cv[kSqU];
 #pragma unroll
     for (int u = 0; u < kSqU; ++u) {
-      const int i = i0 + u;
-      if (live && i < na) {
-        kv[u] = bp_[(int64_t)i * N];
-        wv[u] = wp[(int64_t)i * N];
-      } else {
-        kv[u] = kNoBin;
-      }
+      const int i = min(i0 + u, na - 1);
+      kv[u] = bp_[(int64_t)i * N];
+      wv[u] = wp[(int64_t)i * N];
+      cv[u] = cst[i];
     }
 #pragma unroll
     for (int u = 0; u < kSqU; ++u) {
-      const int k = (int)kv[u];
+      const int k = (live && i0 + u < na) ? (int)kv[u] : kNoBin;
       if (k >= rlo && k < rhi) {
-        const float cc = cst[i0 + u];
         float2 acc = s_acc[k * W + c];
-        acc.x += wv[u].x * cc;
-        acc.y += wv[u].y * cc;
+        acc.x += wv[u].x * cv[u];
+        acc.y += wv[u].y * cv[u];
         s_acc[k * W + c] = acc;
       }
     }

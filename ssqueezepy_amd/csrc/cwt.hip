@@ -112,14 +112,15 @@ __device__ __forceinline__ void emit(const CwtArgs& a, int b, int i, int64_t n,
   }
 }
 
-// X = psih(s xi_k) xh[k] and Xd = X * i xi_k / dt for a positive bin k.
-__device__ __forceinline__ void band_value(const CwtArgs& a, const float2* xh,
-                                           double s, int k, float2& X, float2& Xd) {
+// X = psih(s xi_k) v and Xd = X * i xi_k / dt for a positive bin k, where
+// v = xh[k] was loaded by the caller (loads are kept out of branches so that
+// they are issued together).  valid = false gives zeros.
+__device__ __forceinline__ void band_value(const CwtArgs& a, float2 v, double s, int k,
+                                           bool valid, float2& X, float2& Xd) {
   const int half = a.L >> 1;
   const double xi = (double)k * (kTwoPi / (double)a.L);
-  double ps = psih_eval(a.wav, s * xi);
+  double ps = valid ? psih_eval(a.wav, s * xi) : 0.0;
   if (k == half) ps *= 0.5;                 // Nyquist halving (wavelets.py:86-95)
-  const float2 v = xh[k];
   const float pf = (float)ps;
   X = make_float2(pf * v.x, pf * v.y);
   const float q = (float)(xi * a.inv_dt);
@@ -142,43 +143,77 @@ __global__ __launch_bounds__(kNT) void k_cwt_band(CwtArgs a, const int4* __restr
   const double s = a.scales[i];
 
   // band samples into row 0
-  for (int m = threadIdx.x; m < M; m += kNT) {
-    float2 X = make_float2(0.f, 0.f), Xd = X;
-    const int k = ka + m;
-    if (k <= half) band_value(a, xh, s, k, X, Xd);
-    sW[Lds<M>::idx(0, m)] = X;
-    sD[Lds<M>::idx(0, m)] = Xd;
+  constexpr int IM = (M + kNT - 1) / kNT;
+  {
+    float2 v[IM];
+#pragma unroll
+    for (int it = 0; it < IM; ++it) {
+      const int m = threadIdx.x + it * kNT;
+      v[it] = xh[min(ka + m, half)];
+    }
+#pragma unroll
+    for (int it = 0; it < IM; ++it) {
+      const int m = threadIdx.x + it * kNT;
+      if (m < M) {
+        float2 X, Xd;
+        band_value(a, v[it], s, ka + m, ka + m <= half, X, Xd);
+        sW[Lds<M>::idx(0, m)] = X;
+        sD[Lds<M>::idx(0, m)] = Xd;
+      }
+    }
   }
   __syncthreads();
   // phase ramps w_L^{m (a0 + r)}: rows r >= 1 first, then row 0 in place
-  for (int e = threadIdx.x + M; e < ROWS * M; e += kNT) {
-    const int r = e / M, m = e - r * M;
-    const float2 ph = phase_tab((uint64_t)m * (uint64_t)(a0 + r), a.ptab, L, a.pS);
-    sW[Lds<M>::idx(r, m)] = cmul(sW[Lds<M>::idx(0, m)], ph);
-    sD[Lds<M>::idx(r, m)] = cmul(sD[Lds<M>::idx(0, m)], ph);
-  }
-  __syncthreads();
-  if (a0 != 0) {
-    for (int m = threadIdx.x; m < M; m += kNT) {
-      const float2 ph = phase_tab((uint64_t)m * (uint64_t)a0, a.ptab, L, a.pS);
-      sW[Lds<M>::idx(0, m)] = cmul(sW[Lds<M>::idx(0, m)], ph);
-      sD[Lds<M>::idx(0, m)] = cmul(sD[Lds<M>::idx(0, m)], ph);
+  constexpr int IR = (ROWS * M + kNT - 1) / kNT;
+  {
+    float2 ph[IR];
+#pragma unroll
+    for (int it = 0; it < IR; ++it) {
+      const int e = threadIdx.x + it * kNT;
+      const int r = e / M, m = e - r * M;
+      ph[it] = phase_tab((uint64_t)m * (uint64_t)(a0 + r), a.ptab, L, a.pS);
+    }
+#pragma unroll
+    for (int it = 0; it < IR; ++it) {
+      const int e = threadIdx.x + it * kNT;
+      const int r = e / M, m = e - r * M;
+      if (r >= 1 && r < ROWS) {
+        sW[Lds<M>::idx(r, m)] = cmul(sW[Lds<M>::idx(0, m)], ph[it]);
+        sD[Lds<M>::idx(r, m)] = cmul(sD[Lds<M>::idx(0, m)], ph[it]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IR; ++it) {
+      const int e = threadIdx.x + it * kNT;
+      if (e < M) {
+        sW[Lds<M>::idx(0, e)] = cmul(sW[Lds<M>::idx(0, e)], ph[it]);
+        sD[Lds<M>::idx(0, e)] = cmul(sD[Lds<M>::idx(0, e)], ph[it]);
+      }
     }
   }
   float2* const bufs[2] = {sW, sD};
   lds_fft<M, ROWS, +1, kNT, 2>(bufs, tw);
 
   const float invL = 1.0f / (float)L;
-  for (int e = threadIdx.x; e < ROWS * M; e += kNT) {
+  float2 ph[IR];
+#pragma unroll
+  for (int it = 0; it < IR; ++it) {
+    const int e = threadIdx.x + it * kNT;
+    const int r = e % ROWS, q = e / ROWS;
+    const int64_t n = a0 + r + (int64_t)D * q;
+    ph[it] = phase_tab((uint64_t)ka * (uint64_t)n, a.ptab, L, a.pS);
+  }
+#pragma unroll
+  for (int it = 0; it < IR; ++it) {
+    const int e = threadIdx.x + it * kNT;
     const int r = e % ROWS, q = e / ROWS;
     const int ph_a = a0 + r;
-    if (ph_a >= D) continue;
     const int64_t n = ph_a + (int64_t)D * q;
-    if (n < a.n1 || n >= (int64_t)a.n1 + a.N) continue;
-    float2 ph = phase_tab((uint64_t)ka * (uint64_t)n, a.ptab, L, a.pS);
-    ph.x *= invL;
-    ph.y *= invL;
-    emit(a, b, i, n, cmul(sW[Lds<M>::idx(r, q)], ph), cmul(sD[Lds<M>::idx(r, q)], ph));
+    if (e < ROWS * M && ph_a < D && n >= a.n1 && n < (int64_t)a.n1 + a.N) {
+      const float2 p = make_float2(ph[it].x * invL, ph[it].y * invL);
+      emit(a, b, i, n, cmul(sW[Lds<M>::idx(r, q)], p), cmul(sD[Lds<M>::idx(r, q)], p));
+    }
   }
 }
 
@@ -203,24 +238,49 @@ __global__ __launch_bounds__(kNT) void k_cwt_fullA(CwtArgs a, const int4* __rest
   const int2 kab = band[i];
   const float2* xh = a.xh + (int64_t)b * a.xh_stride;
   const double s = a.scales[i];
-  for (int e = threadIdx.x; e < G * L2; e += kNT) {
-    const int g = e % G, k2 = e / G;
-    const int k = k1_0 + g + L1 * k2;
-    float2 X = make_float2(0.f, 0.f), Xd = X;
-    if (k >= kab.x && k <= kab.y) band_value(a, xh, s, k, X, Xd);
-    sW[Lds<L2>::idx(g, k2)] = X;
-    sD[Lds<L2>::idx(g, k2)] = Xd;
+  constexpr int IA = (G * L2 + kNT - 1) / kNT;
+  {
+    float2 v[IA];
+#pragma unroll
+    for (int it = 0; it < IA; ++it) {
+      const int e = threadIdx.x + it * kNT;
+      const int g = e % G, k2 = e / G;
+      const int k = k1_0 + g + L1 * k2;
+      v[it] = xh[min(max(k, kab.x), max(kab.y, kab.x))];
+    }
+#pragma unroll
+    for (int it = 0; it < IA; ++it) {
+      const int e = threadIdx.x + it * kNT;
+      const int g = e % G, k2 = e / G;
+      const int k = k1_0 + g + L1 * k2;
+      if (e < G * L2) {
+        float2 X = make_float2(0.f, 0.f), Xd = X;
+        const bool in = k >= kab.x && k <= kab.y;
+        if (in) band_value(a, v[it], s, k, true, X, Xd);
+        sW[Lds<L2>::idx(g, k2)] = X;
+        sD[Lds<L2>::idx(g, k2)] = Xd;
+      }
+    }
   }
   float2* const bufs[2] = {sW, sD};
   lds_fft<L2, G, +1, kNT, 2>(bufs, tw);
   const int64_t base = ((int64_t)slot * batch + b) * (int64_t)L2;
-  for (int e = threadIdx.x; e < G * L2; e += kNT) {
+  float2 ph[IA];
+#pragma unroll
+  for (int it = 0; it < IA; ++it) {
+    const int e = threadIdx.x + it * kNT;
     const int g = e % G, n2 = e / G;
-    const int k1 = k1_0 + g;
-    const float2 ph = phase_tab((uint64_t)k1 * (uint64_t)n2, a.ptab, a.L, a.pS);
-    const int64_t o = (base + n2) * L1 + k1;
-    scrW[o] = cmul(sW[Lds<L2>::idx(g, n2)], ph);
-    scrD[o] = cmul(sD[Lds<L2>::idx(g, n2)], ph);
+    ph[it] = phase_tab((uint64_t)(k1_0 + g) * (uint64_t)n2, a.ptab, a.L, a.pS);
+  }
+#pragma unroll
+  for (int it = 0; it < IA; ++it) {
+    const int e = threadIdx.x + it * kNT;
+    const int g = e % G, n2 = e / G;
+    if (e < G * L2) {
+      const int64_t o = (base + n2) * L1 + k1_0 + g;
+      scrW[o] = cmul(sW[Lds<L2>::idx(g, n2)], ph[it]);
+      scrD[o] = cmul(sD[Lds<L2>::idx(g, n2)], ph[it]);
+    }
   }
 }
 
@@ -295,28 +355,46 @@ __global__ __launch_bounds__(256) void k_cwt_os(CwtArgs a, const OsGroup* __rest
   const int64_t j_base = (int64_t)t.y * g.B;              // first unpadded output
   const int64_t start = (int64_t)a.n1 + j_base - g.H;     // padded index of seg[0]
   const float* xb = x + (int64_t)b * x_stride;
-  for (int e = threadIdx.x; e < F; e += NT) {
-    int64_t n = (start + e) % L;
-    if (n < 0) n += L;
-    const int64_t src = pad_src(n, a.N, a.n1, pad);
-    sS[Lds<F>::idx(0, e)] = make_float2(src >= 0 ? xb[src] : 0.f, 0.f);
+  constexpr int IS = (F + NT - 1) / NT;
+  {
+    float xv[IS];
+#pragma unroll
+    for (int it = 0; it < IS; ++it) {
+      const int e = threadIdx.x + it * NT;
+      int64_t n = (start + e) % L;
+      if (n < 0) n += L;
+      const int64_t src = pad_src(n, a.N, a.n1, pad);
+      xv[it] = xb[src >= 0 ? src : 0] * (src >= 0 ? 1.f : 0.f);
+    }
+#pragma unroll
+    for (int it = 0; it < IS; ++it) {
+      const int e = threadIdx.x + it * NT;
+      if (e < F) sS[Lds<F>::idx(0, e)] = make_float2(xv[it], 0.f);
+    }
   }
   {
     float2* const bufs[1] = {sS};
     lds_fft<F, 1, -1, NT, 1>(bufs, tw);
   }
   for (int s0 = 0; s0 < g.ns; s0 += SR) {
-    for (int e = threadIdx.x; e < SR * F; e += NT) {
+    constexpr int IC = (SR * F + NT - 1) / NT;
+    float2 cA[IC], cD[IC];
+#pragma unroll
+    for (int it = 0; it < IC; ++it) {
+      const int e = threadIdx.x + it * NT;
       const int r = e / F, k = e - r * F;
-      float2 A = make_float2(0.f, 0.f), Dv = A;
-      if (s0 + r < g.ns) {
-        const float2 S = sS[Lds<F>::idx(0, k)];
-        const float2* c = coef + (int64_t)(g.rank + s0 + r) * coef_stride;
-        A = cmul(S, c[k]);
-        Dv = cmul(S, c[F + k]);
-      }
-      sA[Lds<F>::idx(r, k)] = A;
-      sD[Lds<F>::idx(r, k)] = Dv;
+      const float2* c = coef + (int64_t)(g.rank + min(s0 + r, g.ns - 1)) * coef_stride;
+      cA[it] = c[k];
+      cD[it] = c[F + k];
+    }
+#pragma unroll
+    for (int it = 0; it < IC; ++it) {
+      const int e = threadIdx.x + it * NT;
+      const int r = e / F, k = e - r * F;
+      const bool ok = s0 + r < g.ns;
+      const float2 S = sS[Lds<F>::idx(0, k)];
+      sA[Lds<F>::idx(r, k)] = ok ? cmul(S, cA[it]) : make_float2(0.f, 0.f);
+      sD[Lds<F>::idx(r, k)] = ok ? cmul(S, cD[it]) : make_float2(0.f, 0.f);
     }
     float2* const bufs[2] = {sA, sD};
     lds_fft<F, SR, +1, NT, 2>(bufs, tw);

@@ -61,7 +61,7 @@ def edge_distance(w_ref, params):
     return np.minimum(np.abs(u - np.floor(u) - 0.5) * d * np.log(2), tr)
 
 
-def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params):
+def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params, axis=-1):
     """Classify squeeze-row mismatches.
 
     edge:  the float64 phase lies within 1e-6 (relative) of a bin edge;
@@ -72,14 +72,16 @@ def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params):
            eps = 1e-6 (5x the measured rel-L2 of Wx); points whose |Wx| is
            within eps * max|Wx| of gamma may also switch sides of gamma;
     other: anything else (must not happen).
+    ``axis`` is the axis of one FFT's outputs: rows of a CWT, columns
+    (frames) of an STFT.
     Returns (n_points, n_mismatch, n_edge, n_noise, n_other)."""
     eps = 1e-6
     mism = k_gpu != k_ref
     w = np.where(np.isfinite(w_ref), w_ref, 0.)
     dist_abs = edge_distance(w_ref, params) * w
     aW = np.abs(Wx_ref)
-    mW = aW.max(axis=-1, keepdims=True)
-    mD = np.abs(dWx_ref).max(axis=-1, keepdims=True)
+    mW = aW.max(axis=axis, keepdims=True)
+    mD = np.abs(dWx_ref).max(axis=axis, keepdims=True)
     with np.errstate(divide='ignore', invalid='ignore'):
         dw = eps * (mD + np.abs(dWx_ref) / aW * mW) / (2 * np.pi * aW)
     near_gamma = np.abs(aW - GAMMA) < eps * mW
@@ -102,11 +104,11 @@ def product_bins(Wx, dWx, bp, strict=True, w=None):
     return k
 
 
-def check_ssq(tag, Tx, Wx, k, Tx_o, Wx_o, dWx_o, w_o, k_o, const, params):
+def check_ssq(tag, Tx, Wx, k, Tx_o, Wx_o, dWx_o, w_o, k_o, const, params, axis=-1):
     """Bins classified against the oracle's; Tx compared raw and on the
     product's own bins (index-matched: the oracle's float64 Wx squeezed with
     the product's rows)."""
-    n, nm, ne, nn, no = bin_report(k, k_o, w_o, Wx_o, dWx_o, params)
+    n, nm, ne, nn, no = bin_report(k, k_o, w_o, Wx_o, dWx_o, params, axis)
     Tx_im = O.ssqueeze(Wx_o, k, const) if Wx_o.ndim == 2 else None
     e_im = rel_l2(Tx, Tx_im)
     e_raw = rel_l2(Tx, Tx_o)
@@ -158,7 +160,9 @@ def test_ssq_cwt_vs_oracle(N, kind, get_w):
         mW = aW.max(axis=-1, keepdims=True)
         mD = np.abs(dWx_o).max(axis=-1, keepdims=True)
         strong = aW > 1e-3 * mW
-        bound = 1e-6 * (mD + np.abs(dWx_o) / aW * mW) / (2 * np.pi * aW) + 1e-6 * np.abs(w_o)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            bound = (1e-6 * (mD + np.abs(dWx_o) / aW * mW) / (2 * np.pi * aW)
+                     + 1e-6 * np.abs(w_o) + 1e-7)
         assert np.array_equal(np.isfinite(wg[strong]), np.isfinite(w_o[strong]))
         assert np.all(np.abs(wg - w_o)[strong] <= bound[strong])
         k = product_bins(Wx, dWx, g['bp'], strict=False, w=w)

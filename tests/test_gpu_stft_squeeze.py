@@ -59,7 +59,8 @@ def test_ssq_stft_vs_oracle(N, n_fft):
     w_f = np.abs(Sfs_o[:, None] - (B * C - A * D) / ((C**2 + D**2) * O.TWO_PI))
     w_f[~(np.abs(Sx_o) > gamma)] = np.inf
     check_ssq(f"ssq_stft N={N} n_fft={n_fft}", Tx.cpu().numpy(), Sx.cpu().numpy(), k,
-              Tx_o, Sx_o, dSx_o, w_f, k_o, Sfs_o[1] - Sfs_o[0], O.bin_params(Sfs_o, False))
+              Tx_o, Sx_o, dSx_o, w_f, k_o, Sfs_o[1] - Sfs_o[0], O.bin_params(Sfs_o, False),
+              axis=0)
 
 
 def test_ssq_stft_vs_reference_golden(golden):
