@@ -172,7 +172,8 @@ constexpr int kSqW = 64;
 constexpr int kSqNT = 256;
 constexpr int kSqU = 32;
 
-__global__ __launch_bounds__(kSqNT) void k_squeeze(const float2* __restrict__ Wx,
+__global__ __launch_bounds__(kSqNT) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_squeeze(const float2* __restrict__ Wx,
                                                    const uint16_t* __restrict__ bins,
                                                    const float* __restrict__ cst, int na,
                                                    int nf, int64_t N, int64_t in_stride,
@@ -197,28 +198,41 @@ __global__ __launch_bounds__(kSqNT) void k_squeeze(const float2* __restrict__ Wx
   const float2* wp = Wx + (int64_t)b * in_stride + (live ? j : 0);
   const uint16_t* bp_ = bins + (int64_t)b * in_stride + (live ? j : 0);
 
+  // double-buffered: the loads of chunk i0 + kSqU are in flight while the
+  // accumulators take chunk i0; loads are unconditional (clamped rows) so
+  // the compiler issues them back to back, invalid rows are masked
+  float2 wA[kSqU];
+  uint32_t kA[kSqU];
+#pragma unroll
+  for (int u = 0; u < kSqU; ++u) {
+    const int i = min(u, na - 1);
+    kA[u] = bp_[(int64_t)i * N];
+    wA[u] = wp[(int64_t)i * N];
+  }
   for (int i0 = 0; i0 < na; i0 += kSqU) {
-    // unconditional (clamped) loads so the compiler issues all kSqU of them
-    // before the first use; invalid rows are masked afterwards
-    float2 wv[kSqU];
-    uint32_t kv[kSqU];
-    float cv[kSqU];
+    float2 wB[kSqU];
+    uint32_t kB[kSqU];
 #pragma unroll
     for (int u = 0; u < kSqU; ++u) {
-      const int i = min(i0 + u, na - 1);
-      kv[u] = bp_[(int64_t)i * N];
-      wv[u] = wp[(int64_t)i * N];
-      cv[u] = cst[i];
+      const int i = min(i0 + kSqU + u, na - 1);
+      kB[u] = bp_[(int64_t)i * N];
+      wB[u] = wp[(int64_t)i * N];
     }
 #pragma unroll
     for (int u = 0; u < kSqU; ++u) {
-      const int k = (live && i0 + u < na) ? (int)kv[u] : kNoBin;
+      const int k = (live && i0 + u < na) ? (int)kA[u] : kNoBin;
       if (k >= rlo && k < rhi) {
+        const float cc = cst[i0 + u];
         float2 acc = s_acc[k * W + c];
-        acc.x += wv[u].x * cv[u];
-        acc.y += wv[u].y * cv[u];
+        acc.x += wA[u].x * cc;
+        acc.y += wA[u].y * cc;
         s_acc[k * W + c] = acc;
       }
+    }
+#pragma unroll
+    for (int u = 0; u < kSqU; ++u) {
+      kA[u] = kB[u];
+      wA[u] = wB[u];
     }
   }
   __syncthreads();

@@ -29,13 +29,19 @@ using namespace ssqk;
 
 namespace {
 
-constexpr int kP = 8192;    // complex points per workgroup and channel
-constexpr int kNT = 512;    // threads per FFT workgroup
+constexpr int kP = 8192;    // largest band transform of one workgroup
+constexpr int kNT = 512;    // threads of the single-workgroup forward FFT
 constexpr int kSmallFwd = 16384;  // largest L done by one workgroup
+constexpr int kPF = 4096;   // points per channel of a four-step workgroup
 
-// phases per workgroup of the band path: kP / M, capped so the row padding of
-// tiny transforms stays within LDS
-constexpr int band_rows(int M) { return kP / M > 1024 ? 1024 : kP / M; }
+// Points per channel of a band-path workgroup: at least 2048 so that small
+// transforms batch many phases, at most M so that LDS stays <= 70 KB for
+// M <= 4096 (two or more workgroups per CU overlap their load / FFT /
+// store phases).  Threads: one radix-16 butterfly per thread per pass
+// (two channels).
+constexpr int band_points(int M) { return M > 2048 ? M : 2048; }
+constexpr int band_rows(int M) { return band_points(M) / M > 1024 ? 1024 : band_points(M) / M; }
+constexpr int band_threads(int M) { return band_points(M) / 8; }
 
 struct CwtArgs {
   const float2* xh;
@@ -130,9 +136,11 @@ __device__ __forceinline__ void band_value(const CwtArgs& a, float2 v, double s,
 // ------------------------------------------------------------- band path
 // task = {scale i, first phase a0, band start ka, log2 D}
 template <int M>
-__global__ __launch_bounds__(kNT) void k_cwt_band(CwtArgs a, const int4* __restrict__ tasks,
-                                                  const float2* __restrict__ tw) {
+__global__ __launch_bounds__(band_threads(M)) void k_cwt_band(CwtArgs a,
+                                                             const int4* __restrict__ tasks,
+                                                             const float2* __restrict__ tw) {
   constexpr int ROWS = band_rows(M);
+  constexpr int kNT = band_threads(M);
   __shared__ float2 sW[Lds<M>::elems(ROWS)];
   __shared__ float2 sD[Lds<M>::elems(ROWS)];
   const int4 t = tasks[blockIdx.x];
@@ -224,12 +232,13 @@ __global__ __launch_bounds__(kNT) void k_cwt_band(CwtArgs a, const int4* __restr
 // step B (rows = consecutive n2): FFT over k1, output stage.
 // taskA = {i, k1_0, slot, -}, taskB = {i, n2_0, slot, -}; band[i] = {ka, kb}.
 template <int L2>
-__global__ __launch_bounds__(kNT) void k_cwt_fullA(CwtArgs a, const int4* __restrict__ tasks,
+__global__ __launch_bounds__(kPF / 8) void k_cwt_fullA(CwtArgs a, const int4* __restrict__ tasks,
                                                    const int2* __restrict__ band,
                                                    const float2* __restrict__ tw, int L1,
                                                    int batch, float2* __restrict__ scrW,
                                                    float2* __restrict__ scrD) {
-  constexpr int G = kP / L2;
+  constexpr int G = kPF / L2;
+  constexpr int kNT = kPF / 8;
   __shared__ float2 sW[Lds<L2>::elems(G)];
   __shared__ float2 sD[Lds<L2>::elems(G)];
   const int4 t = tasks[blockIdx.x];
@@ -285,11 +294,12 @@ __global__ __launch_bounds__(kNT) void k_cwt_fullA(CwtArgs a, const int4* __rest
 }
 
 template <int L1>
-__global__ __launch_bounds__(kNT) void k_cwt_fullB(CwtArgs a, const int4* __restrict__ tasks,
+__global__ __launch_bounds__(kPF / 8) void k_cwt_fullB(CwtArgs a, const int4* __restrict__ tasks,
                                                    const float2* __restrict__ tw, int L2,
                                                    int batch, const float2* __restrict__ scrW,
                                                    const float2* __restrict__ scrD) {
-  constexpr int G = kP / L1;
+  constexpr int G = kPF / L1;
+  constexpr int kNT = kPF / 8;
   __shared__ float2 sW[Lds<L1>::elems(G)];
   __shared__ float2 sD[Lds<L1>::elems(G)];
   const int4 t = tasks[blockIdx.x];
@@ -332,11 +342,11 @@ struct OsGroup {
 template <int F>
 struct OsShape {
   static constexpr int SR = (4096 / F) > 8 ? 8 : (4096 / F);  // scales per batch
-  static constexpr int NT = 256;
+  static constexpr int NT = (SR * F) / 8 < 64 ? 64 : (SR * F) / 8;
 };
 
 template <int F>
-__global__ __launch_bounds__(256) void k_cwt_os(CwtArgs a, const OsGroup* __restrict__ groups,
+__global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGroup* __restrict__ groups,
                                                  const int2* __restrict__ tasks,
                                                  const float2* __restrict__ coef,
                                                  int coef_stride,
@@ -345,7 +355,8 @@ __global__ __launch_bounds__(256) void k_cwt_os(CwtArgs a, const OsGroup* __rest
                                                  int pad) {
   constexpr int SR = OsShape<F>::SR;
   constexpr int NT = OsShape<F>::NT;
-  __shared__ float2 sS[Lds<F>::elems(1)];
+  // two channels x SR rows in LDS (<= 70 KB); the segment spectrum stays in
+  // registers, in the layout of the multiply stage
   __shared__ float2 sA[Lds<F>::elems(SR)];
   __shared__ float2 sD[Lds<F>::elems(SR)];
   const int2 t = tasks[blockIdx.x];
@@ -369,32 +380,31 @@ __global__ __launch_bounds__(256) void k_cwt_os(CwtArgs a, const OsGroup* __rest
 #pragma unroll
     for (int it = 0; it < IS; ++it) {
       const int e = threadIdx.x + it * NT;
-      if (e < F) sS[Lds<F>::idx(0, e)] = make_float2(xv[it], 0.f);
+      if (e < F) sA[Lds<F>::idx(0, e)] = make_float2(xv[it], 0.f);
     }
   }
   {
-    float2* const bufs[1] = {sS};
+    float2* const bufs[1] = {sA};
     lds_fft<F, 1, -1, NT, 1>(bufs, tw);
   }
+  constexpr int IC = (SR * F + NT - 1) / NT;
+  float2 Sreg[IC];
+#pragma unroll
+  for (int it = 0; it < IC; ++it) {
+    const int e = threadIdx.x + it * NT;
+    Sreg[it] = sA[Lds<F>::idx(0, e % F)];
+  }
+  __syncthreads();
   for (int s0 = 0; s0 < g.ns; s0 += SR) {
-    constexpr int IC = (SR * F + NT - 1) / NT;
-    float2 cA[IC], cD[IC];
 #pragma unroll
     for (int it = 0; it < IC; ++it) {
       const int e = threadIdx.x + it * NT;
       const int r = e / F, k = e - r * F;
       const float2* c = coef + (int64_t)(g.rank + min(s0 + r, g.ns - 1)) * coef_stride;
-      cA[it] = c[k];
-      cD[it] = c[F + k];
-    }
-#pragma unroll
-    for (int it = 0; it < IC; ++it) {
-      const int e = threadIdx.x + it * NT;
-      const int r = e / F, k = e - r * F;
+      const float2 cA = c[k], cD = c[F + k];
       const bool ok = s0 + r < g.ns;
-      const float2 S = sS[Lds<F>::idx(0, k)];
-      sA[Lds<F>::idx(r, k)] = ok ? cmul(S, cA[it]) : make_float2(0.f, 0.f);
-      sD[Lds<F>::idx(r, k)] = ok ? cmul(S, cD[it]) : make_float2(0.f, 0.f);
+      sA[Lds<F>::idx(r, k)] = ok ? cmul(Sreg[it], cA) : make_float2(0.f, 0.f);
+      sD[Lds<F>::idx(r, k)] = ok ? cmul(Sreg[it], cD) : make_float2(0.f, 0.f);
     }
     float2* const bufs[2] = {sA, sD};
     lds_fft<F, SR, +1, NT, 2>(bufs, tw);
@@ -529,7 +539,7 @@ int launch_band(int M, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* 
   switch (M) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_cwt_band<m>, grid, dim3(kNT), 0, st, a, tasks,            \
+    hipLaunchKernelGGL(k_cwt_band<m>, grid, dim3(band_threads(m)), 0, st, a, tasks, \
                        tw);                                    \
     break;
     SSQK_POW2_CASES(C)
@@ -547,7 +557,7 @@ int launch_fullA(int L2, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
   switch (L2) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_cwt_fullA<m>, grid, dim3(kNT), 0, st, a, tasks, band,     \
+    hipLaunchKernelGGL(k_cwt_fullA<m>, grid, dim3(kPF / 8), 0, st, a, tasks, band, \
                        tw, L1, batch, sW, sD);                 \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
@@ -565,7 +575,7 @@ int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
   switch (L1) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_cwt_fullB<m>, grid, dim3(kNT), 0, st, a, tasks,           \
+    hipLaunchKernelGGL(k_cwt_fullB<m>, grid, dim3(kPF / 8), 0, st, a, tasks,       \
                        tw, L2, batch, sW, sD);                 \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
@@ -586,7 +596,8 @@ int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup*
   switch (F) {
 #define C(m)                                                                        \
   case m:                                                                           \
-    hipLaunchKernelGGL(k_cwt_os<m>, grid, dim3(256), 0, st, a, groups, tasks, coef, \
+    hipLaunchKernelGGL(k_cwt_os<m>, grid, dim3(OsShape<m>::NT), 0, st, a, groups,   \
+                       tasks, coef,                                                 \
                        coef_stride, tw, x, x_stride, pad);                          \
     break;
     SSQK_OS_CASES(C)
@@ -977,7 +988,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
     const int64_t per_slot = L * 2 * (int64_t)sizeof(float2);
     p->slots = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)p->full.size(),
                                                             (64ll << 20) / per_slot));
-    const int GA = kP / p->L2, GB = kP / p->L1;
+    const int GA = kPF / p->L2, GB = kPF / p->L1;
     for (size_t c0 = 0; c0 < p->full.size() && e == hipSuccess; c0 += p->slots) {
       std::vector<int4> A, B;
       for (int sl = 0; sl < p->slots && c0 + sl < p->full.size(); ++sl) {

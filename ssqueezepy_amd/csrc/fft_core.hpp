@@ -185,57 +185,65 @@ struct Plan {
   }
 };
 
+// buf[c] for a run-time channel index without materializing the pointer
+// array (LDS pointers cannot live in a static initializer)
+template <int NCH>
+__device__ __forceinline__ float2* pick(float2* const (&buf)[NCH], int c) {
+  static_assert(NCH == 1 || NCH == 2, "one or two channels");
+  if constexpr (NCH == 1) {
+    return buf[0];
+  } else {
+    return c == 0 ? buf[0] : buf[1];
+  }
+}
+
 // One Stockham pass of radix R over ROWS rows of length M in LDS, applied
-// to NCH channels (buffers) at once.  tw[t] = exp(+2 pi i t / M).
-// Each thread loads all of its butterflies' inputs before the barrier and
-// writes after it, so the pass can run in place.
+// to NCH channels (buffers).  tw[t] = exp(+2 pi i t / M).  Work items are
+// (channel, butterfly) pairs spread over the NT threads, so a workgroup with
+// NT = NCH * ROWS * M / R threads does one butterfly per thread.  Each thread
+// loads its inputs before the barrier and writes after it, so the pass runs
+// in place.
 template <int M, int ROWS, int R, int NS, int DIR, int NT, int NCH>
 __device__ __forceinline__ void stockham_pass(float2* const (&buf)[NCH],
                                               const float2* __restrict__ tw) {
   constexpr int BPR = M / R;                 // butterflies per row
-  constexpr int NBF = ROWS * BPR;
-  constexpr int ITER = (NBF + NT - 1) / NT;
-  float2 v[NCH][ITER][R];
+  constexpr int NBF = ROWS * BPR;            // butterflies per channel
+  constexpr int NIT = NCH * NBF;
+  constexpr int ITER = (NIT + NT - 1) / NT;
+  float2 v[ITER][R];
   const int tid = threadIdx.x;
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
-    const int j = tid + it * NT;
-    if ((NBF % NT == 0) || j < NBF) {
+    const int wi = tid + it * NT;
+    if ((NIT % NT == 0) || wi < NIT) {
+      const int c = wi / NBF, j = wi - c * NBF;
       const int row = j / BPR, jj = j - row * BPR;
       const int k = jj % NS;
-      float2 w[R];
+      float2* const b = pick(buf, c);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = b[Lds<M>::idx(row, jj + r * BPR)];
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 t = tw[(r * k) * (M / (NS * R))];
-          w[r] = DIR > 0 ? t : cconj(t);
+          const float2 t = tw[(r * k) * (M / (NS * R))];
+          v[it][r] = cmul(v[it][r], DIR > 0 ? t : cconj(t));
         }
       }
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[c][it][r] = buf[c][Lds<M>::idx(row, jj + r * BPR)];
-        if constexpr (NS > 1) {
-#pragma unroll
-          for (int r = 1; r < R; ++r) v[c][it][r] = cmul(v[c][it][r], w[r]);
-        }
-        RegDFT<R, DIR>::run(v[c][it]);
-      }
+      RegDFT<R, DIR>::run(v[it]);
     }
   }
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
-    const int j = tid + it * NT;
-    if ((NBF % NT == 0) || j < NBF) {
+    const int wi = tid + it * NT;
+    if ((NIT % NT == 0) || wi < NIT) {
+      const int c = wi / NBF, j = wi - c * NBF;
       const int row = j / BPR, jj = j - row * BPR;
       const int k = jj % NS;
       const int base = (jj - k) * R + k;
+      float2* const b = pick(buf, c);
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) buf[c][Lds<M>::idx(row, base + r * NS)] = v[c][it][r];
-      }
+      for (int r = 0; r < R; ++r) b[Lds<M>::idx(row, base + r * NS)] = v[it][r];
     }
   }
   __syncthreads();

@@ -88,6 +88,10 @@ def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params, axis=-1):
     edge = mism & (edge_distance(w_ref, params) < 1e-6)
     noise = mism & ~edge & ((dist_abs < dw) | near_gamma | ~np.isfinite(dw))
     other = mism & ~edge & ~noise
+    for idx in list(zip(*np.nonzero(other)))[:8]:
+        print("  unexplained mismatch at", idx, "rows", k_gpu[idx], k_ref[idx],
+              "w %.9g" % w_ref[idx], "edge dist %.3g" % edge_distance(w_ref, params)[idx],
+              "|w err| bound %.3g" % dw[idx], "|Wx|/max %.3g" % (aW / mW)[idx])
     return k_ref.size, int(mism.sum()), int(edge.sum()), int(noise.sum()), int(other.sum())
 
 
