@@ -160,87 +160,120 @@ __global__ void k_bins_w(const float* __restrict__ w, int64_t total, BinDev bp,
 }
 
 // -------------------------------------------------------------- squeeze
-// One workgroup owns W consecutive columns of one signal for every scale.
-// Wave v handles columns [v W/4, (v+1) W/4); the lanes of a column split the
-// output rows into equal slices, so every (row, column) accumulator has
-// exactly one owner lane and each column is summed in ascending scale order
-// (deterministic, no atomics).  The lanes of a column load the same
-// addresses, which the memory pipeline serves once.  Accumulators are
-// interleaved complex [nf][W] in LDS; kSqU scales are loaded ahead so each
-// wave keeps enough HBM requests in flight.
-constexpr int kSqW = 64;
-constexpr int kSqNT = 256;
-constexpr int kSqU = 32;
+// One workgroup owns W consecutive columns of one signal and a window of
+// output rows [r0, r0 + R) for every scale; the accumulators are
+// interleaved complex [R + 4][W + 1] in LDS (odd row stride; 4 dummy rows).
+// Eight lanes serve one column: two quads, each lane loading its own scale,
+// so one load instruction covers 8 columns x 8 consecutive scales.  Scales
+// are taken in groups of 8: quad p holds scales 4p..4p+3 of the group.
+// Inside a quad the four (bin, value) pairs are exchanged by DPP
+// broadcasts; contributions that land in the same row are summed in
+// ascending scale order by the lowest such lane, the others target their
+// dummy row.  Quad 0 then quad 1 update LDS, so every accumulator receives
+// its column's scales in ascending order: deterministic, no atomics, Tx
+// written once.  kSqUL groups are loaded ahead of the ones being summed.
+constexpr int kSqQPC = 2;                    // quads per column
+constexpr int kSqGS = 4 * kSqQPC;            // scales per group
+constexpr int kSqUL = 4;                     // groups per prefetch block
+constexpr int kSqCPW = 16 / kSqQPC;          // columns per wave
 
-__global__ __launch_bounds__(kSqNT) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_squeeze(const float2* __restrict__ Wx,
-                                                   const uint16_t* __restrict__ bins,
-                                                   const float* __restrict__ cst, int na,
-                                                   int nf, int64_t N, int64_t in_stride,
-                                                   int64_t out_stride, int W,
-                                                   float2* __restrict__ Tx) {
-  extern __shared__ float2 s_acc[];          // [nf][W]
+template <int S>
+__device__ __forceinline__ int quad_bcast(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, S * 0x55, 0xF, 0xF, true);
+}
+template <int S>
+__device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(quad_bcast<S>(__float_as_int(v)));
+}
+
+template <int W>
+__global__ __launch_bounds__(W * 4 * kSqQPC)
+void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
+               const float* __restrict__ cst, int na, int r0, int R, int64_t N,
+               int64_t in_stride, int64_t out_stride, float2* __restrict__ Tx) {
+  constexpr int NT = W * 4 * kSqQPC;
+  constexpr int RS = W + 1;
+  extern __shared__ float2 s_acc[];          // [R + 4][RS]
   const int b = blockIdx.y;
   const int64_t j0 = (int64_t)blockIdx.x * W;
   const int tid = threadIdx.x;
-  for (int e = tid; e < nf * W; e += kSqNT) s_acc[e] = make_float2(0.f, 0.f);
+  for (int e = tid; e < (R + 4) * RS; e += NT) s_acc[e] = make_float2(0.f, 0.f);
   __syncthreads();
 
-  const int lanes_per_col = kSqNT / W;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int cols_per_wave = W / 4;
-  const int c = wave * cols_per_wave + (lane % cols_per_wave);
-  const int q = lane / cols_per_wave;
-  const int rlo = (int)(((int64_t)nf * q) / lanes_per_col);
-  const int rhi = (int)(((int64_t)nf * (q + 1)) / lanes_per_col);
-  const int64_t j = j0 + c;
-  const bool live = j < N;
-  const float2* wp = Wx + (int64_t)b * in_stride + (live ? j : 0);
-  const uint16_t* bp_ = bins + (int64_t)b * in_stride + (live ? j : 0);
+  const int lane = tid & 63, wave = tid >> 6;
+  const int q = lane & 3;
+  const int p = (lane >> 2) % kSqQPC;
+  const int c = wave * kSqCPW + lane / (4 * kSqQPC);
+  const int sl = 4 * p + q;                  // this lane's scale in a group
+  const bool live = j0 + c < N;
+  const uint32_t cl = live ? (uint32_t)c : 0u;
+  const uint32_t offL = (uint32_t)sl * (uint32_t)N + cl;
+  const float2* Wb = Wx + (int64_t)b * in_stride + j0;
+  const uint16_t* Bb = bins + (int64_t)b * in_stride + j0;
+  const int dummy = R + q;
 
-  // double-buffered: the loads of chunk i0 + kSqU are in flight while the
-  // accumulators take chunk i0; loads are unconditional (clamped rows) so
-  // the compiler issues them back to back, invalid rows are masked
-  float2 wA[kSqU];
-  uint32_t kA[kSqU];
+  // rows past na re-read row ib (in bounds) and are masked below
+  auto load = [&](int i0, float2* w, int* k, float* cc) {
 #pragma unroll
-  for (int u = 0; u < kSqU; ++u) {
-    const int i = min(u, na - 1);
-    kA[u] = bp_[(int64_t)i * N];
-    wA[u] = wp[(int64_t)i * N];
-  }
-  for (int i0 = 0; i0 < na; i0 += kSqU) {
-    float2 wB[kSqU];
-    uint32_t kB[kSqU];
-#pragma unroll
-    for (int u = 0; u < kSqU; ++u) {
-      const int i = min(i0 + kSqU + u, na - 1);
-      kB[u] = bp_[(int64_t)i * N];
-      wB[u] = wp[(int64_t)i * N];
+    for (int u = 0; u < kSqUL; ++u) {
+      const int ib = min(i0 + kSqGS * u, na - 1);
+      const bool in = ib + sl < na;
+      const uint32_t off = in ? offL : cl;
+      k[u] = Bb[(int64_t)ib * N + off];
+      w[u] = Wb[(int64_t)ib * N + off];
+      cc[u] = cst[ib + (in ? sl : 0)];
     }
+  };
+  float2 wA[kSqUL];
+  int kA[kSqUL];
+  float cA[kSqUL];
+  load(0, wA, kA, cA);
+  for (int i0 = 0; i0 < na; i0 += kSqGS * kSqUL) {
+    float2 wB[kSqUL];
+    int kB[kSqUL];
+    float cB[kSqUL];
+    load(i0 + kSqGS * kSqUL, wB, kB, cB);
 #pragma unroll
-    for (int u = 0; u < kSqU; ++u) {
-      const int k = (live && i0 + u < na) ? (int)kA[u] : kNoBin;
-      if (k >= rlo && k < rhi) {
-        const float cc = cst[i0 + u];
-        float2 acc = s_acc[k * W + c];
-        acc.x += wA[u].x * cc;
-        acc.y += wA[u].y * cc;
-        s_acc[k * W + c] = acc;
+    for (int u = 0; u < kSqUL; ++u) {
+      const int ib = i0 + kSqGS * u;
+      const int kr = (live && ib + sl < na) ? kA[u] - r0 : -1;
+      const int k = (kr >= 0 && kr < R) ? kr : -1;
+      const float pre = wA[u].x * cA[u], pim = wA[u].y * cA[u];
+      const int k0 = quad_bcast<0>(k), k1 = quad_bcast<1>(k);
+      const int k2 = quad_bcast<2>(k), k3 = quad_bcast<3>(k);
+      const float x0 = quad_bcast<0>(pre), x1 = quad_bcast<1>(pre);
+      const float x2 = quad_bcast<2>(pre), x3 = quad_bcast<3>(pre);
+      const float y0 = quad_bcast<0>(pim), y1 = quad_bcast<1>(pim);
+      const float y2 = quad_bcast<2>(pim), y3 = quad_bcast<3>(pim);
+      const bool dup = (q > 0 && k0 == k) || (q > 1 && k1 == k) || (q > 2 && k2 == k);
+      float sx = 0.f, sy = 0.f;
+      if (q <= 0 && k0 == k) { sx += x0; sy += y0; }
+      if (q <= 1 && k1 == k) { sx += x1; sy += y1; }
+      if (q <= 2 && k2 == k) { sx += x2; sy += y2; }
+      if (k3 == k) { sx += x3; sy += y3; }
+      const int row = (dup || k < 0) ? dummy : k;
+#pragma unroll
+      for (int pp = 0; pp < kSqQPC; ++pp) {
+        float2* a = s_acc + (p == pp ? row : dummy) * RS + c;
+        float2 acc = *a;
+        acc.x += sx;
+        acc.y += sy;
+        *a = acc;
       }
     }
 #pragma unroll
-    for (int u = 0; u < kSqU; ++u) {
+    for (int u = 0; u < kSqUL; ++u) {
       kA[u] = kB[u];
       wA[u] = wB[u];
+      cA[u] = cB[u];
     }
   }
   __syncthreads();
-  float2* tp = Tx + (int64_t)b * out_stride;
-  for (int e = tid; e < nf * W; e += kSqNT) {
+  float2* tp = Tx + (int64_t)b * out_stride + (int64_t)r0 * N;
+  for (int e = tid; e < R * W; e += NT) {
     const int k = e / W, cc = e - k * W;
     const int64_t jj = j0 + cc;
-    if (jj < N) tp[(int64_t)k * N + jj] = s_acc[e];
+    if (jj < N) tp[(int64_t)k * N + jj] = s_acc[k * RS + cc];
   }
 }
 
@@ -331,30 +364,38 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
                             int64_t batch, int64_t na, int64_t nf, int64_t N, void* Tx,
                             void* stream) {
   if (!Wx || !bins || !cst || !Tx || batch < 1 || batch > 65535 || na < 1 || nf < 1 ||
-      nf >= kNoBin || N < 1) {
+      nf >= kNoBin || N < 1 || N * kSqGS >= ((int64_t)1 << 31)) {
     ssqk_host::set_error("bad arguments to ssqk_squeeze");
     return SSQK_ERR_ARG;
   }
-  // accumulators [2][nf][W] in LDS; W = 64 columns while it fits 160 KiB,
-  // narrower for very many output rows
-  int W = kSqW;
-  while (W > 4 && (size_t)nf * W * sizeof(float2) > 160 * 1024) W /= 2;
-  if ((size_t)nf * W * sizeof(float2) > 160 * 1024) {
-    ssqk_host::set_error("nf=%lld output rows do not fit the LDS accumulator",
-                         (long long)nf);
-    return SSQK_ERR_UNSUPPORTED;
-  }
-  const size_t lds = (size_t)nf * W * sizeof(float2);
-  static bool attr_set = false;
-  if (!attr_set) {
-    SSQK_CHECK(hipFuncSetAttribute((const void*)k_squeeze,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_set = true;
-  }
-  dim3 grid((unsigned)((N + W - 1) / W), (unsigned)batch);
-  hipLaunchKernelGGL(k_squeeze, grid, dim3(kSqNT), lds, (hipStream_t)stream,
-                     (const float2*)Wx, bins, cst, (int)na, (int)nf, N, na * N, nf * N, W,
-                     (float2*)Tx);
-  SSQK_CHECK(hipGetLastError());
-  return SSQK_OK;
+  // accumulators [R + 4][W + 1] complex in 160 KiB of LDS: 32 columns while
+  // all rows fit, else 16 or 8; taller outputs are done in row windows
+  constexpr size_t kLds = 160 * 1024;
+  auto fits = [&](int64_t rows, int w) {
+    return (size_t)(rows + 4) * (w + 1) * sizeof(float2) <= kLds;
+  };
+  int W = 32;
+  while (W > 8 && !fits(nf, W)) W /= 2;
+  const int64_t R = fits(nf, W) ? nf : (int64_t)(kLds / ((W + 1) * sizeof(float2))) - 4;
+  static bool attr_set[3] = {false, false, false};
+  auto launch = [&](auto kern, int wi) -> int {
+    if (!attr_set[wi]) {
+      SSQK_CHECK(hipFuncSetAttribute((const void*)kern,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds));
+      attr_set[wi] = true;
+    }
+    for (int64_t r0 = 0; r0 < nf; r0 += R) {
+      const int64_t rows = std::min<int64_t>(R, nf - r0);
+      const size_t lds = (size_t)(rows + 4) * (W + 1) * sizeof(float2);
+      dim3 grid((unsigned)((N + W - 1) / W), (unsigned)batch);
+      hipLaunchKernelGGL(kern, grid, dim3(W * 4 * kSqQPC), lds, (hipStream_t)stream,
+                         (const float2*)Wx, bins, cst, (int)na, (int)r0, (int)rows, N,
+                         na * N, nf * N, (float2*)Tx);
+      SSQK_CHECK(hipGetLastError());
+    }
+    return SSQK_OK;
+  };
+  if (W == 32) return launch(k_squeeze<32>, 0);
+  if (W == 16) return launch(k_squeeze<16>, 1);
+  return launch(k_squeeze<8>, 2);
 }

@@ -61,7 +61,7 @@ def edge_distance(w_ref, params):
     return np.minimum(np.abs(u - np.floor(u) - 0.5) * d * np.log(2), tr)
 
 
-def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params, axis=-1):
+def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params, axis=-1, shared_fft=False):
     """Classify squeeze-row mismatches.
 
     edge:  the float64 phase lies within 1e-6 (relative) of a bin edge;
@@ -73,7 +73,10 @@ def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params, axis=-1):
            within eps * max|Wx| of gamma may also switch sides of gamma;
     other: anything else (must not happen).
     ``axis`` is the axis of one FFT's outputs: rows of a CWT, columns
-    (frames) of an STFT.
+    (frames) of an STFT.  ``shared_fft``: dWx comes out of the same complex
+    FFT as Wx (the STFT packs window and window derivative into one
+    transform and splits the result by Hermitian symmetry), so its error
+    scales with max(|Wx|, |dWx|) rather than with max|dWx| alone.
     Returns (n_points, n_mismatch, n_edge, n_noise, n_other)."""
     eps = 1e-6
     mism = k_gpu != k_ref
@@ -82,6 +85,8 @@ def bin_report(k_gpu, k_ref, w_ref, Wx_ref, dWx_ref, params, axis=-1):
     aW = np.abs(Wx_ref)
     mW = aW.max(axis=axis, keepdims=True)
     mD = np.abs(dWx_ref).max(axis=axis, keepdims=True)
+    if shared_fft:
+        mD = mD + mW
     with np.errstate(divide='ignore', invalid='ignore'):
         dw = eps * (mD + np.abs(dWx_ref) / aW * mW) / (2 * np.pi * aW)
     near_gamma = np.abs(aW - GAMMA) < eps * mW
@@ -108,11 +113,12 @@ def product_bins(Wx, dWx, bp, strict=True, w=None):
     return k
 
 
-def check_ssq(tag, Tx, Wx, k, Tx_o, Wx_o, dWx_o, w_o, k_o, const, params, axis=-1):
+def check_ssq(tag, Tx, Wx, k, Tx_o, Wx_o, dWx_o, w_o, k_o, const, params, axis=-1,
+              shared_fft=False):
     """Bins classified against the oracle's; Tx compared raw and on the
     product's own bins (index-matched: the oracle's float64 Wx squeezed with
     the product's rows)."""
-    n, nm, ne, nn, no = bin_report(k, k_o, w_o, Wx_o, dWx_o, params, axis)
+    n, nm, ne, nn, no = bin_report(k, k_o, w_o, Wx_o, dWx_o, params, axis, shared_fft)
     Tx_im = O.ssqueeze(Wx_o, k, const) if Wx_o.ndim == 2 else None
     e_im = rel_l2(Tx, Tx_im)
     e_raw = rel_l2(Tx, Tx_o)

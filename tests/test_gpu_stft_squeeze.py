@@ -60,7 +60,7 @@ def test_ssq_stft_vs_oracle(N, n_fft):
     w_f[~(np.abs(Sx_o) > gamma)] = np.inf
     check_ssq(f"ssq_stft N={N} n_fft={n_fft}", Tx.cpu().numpy(), Sx.cpu().numpy(), k,
               Tx_o, Sx_o, dSx_o, w_f, k_o, Sfs_o[1] - Sfs_o[0], O.bin_params(Sfs_o, False),
-              axis=0)
+              axis=0, shared_fft=True)
 
 
 def test_ssq_stft_vs_reference_golden(golden):
@@ -109,3 +109,38 @@ def test_phase_entry_points_vs_reference(golden):
     T = algos.ssqueeze_fast(Wx, dWx, s['stft/Sfs'], s['const'], False, False, gamma=1e-2,
                             Sfs=s['stft/Sfs']).cpu().numpy()
     assert rel_l2(T, s['stft/fast']) < 1e-5
+
+
+@pytest.mark.parametrize('na,nf,N,batch', [(1, 1, 1, 1), (7, 5, 33, 2), (293, 293, 1000, 1),
+                                           (300, 3000, 700, 1), (17, 2500, 50, 2),
+                                           (9, 70, 4099, 3)])
+def test_squeeze_kernel_shapes(na, nf, N, batch):
+    """ssqk_squeeze against the float64 brute-force scatter: partial column
+    blocks, scale counts that are not a multiple of the 8-scale groups,
+    empty bins, row windows (output rows beyond one LDS tile), batches;
+    bins clustered so many scales of a column share a row."""
+    import torch
+    from ssqueezepy_amd import _lib
+    rng = np.random.default_rng(na * 7 + nf)
+    Wx = (rng.standard_normal((batch, na, N)) + 1j * rng.standard_normal((batch, na, N)))
+    ridge = rng.integers(0, nf, size=(batch, 1, N))
+    k = np.where(rng.random((batch, na, N)) < 0.5, ridge, rng.integers(0, nf, (batch, na, N)))
+    k[rng.random((batch, na, N)) < 0.1] = -1
+    cst = rng.random(na) + 0.5
+    Wx_d = torch.as_tensor(Wx.astype(np.complex64), device='cuda')
+    bins = torch.as_tensor(np.where(k < 0, 0xFFFF, k).astype(np.uint16).view(np.int16),
+                           device='cuda')
+    cst_d = torch.as_tensor(cst.astype(np.float32), device='cuda')
+    Tx = torch.empty((batch, nf, N), dtype=torch.complex64, device='cuda')
+    print(f"squeeze na={na} nf={nf} N={N} batch={batch}", flush=True)
+    _lib.squeeze(Wx_d, bins, cst_d, nf, Tx)
+    Tx2 = torch.empty_like(Tx)
+    _lib.squeeze(Wx_d, bins, cst_d, nf, Tx2)
+    assert torch.equal(Tx, Tx2)                 # deterministic
+    Wx32 = Wx.astype(np.complex64).astype(np.complex128)
+    c32 = cst.astype(np.float32).astype(np.float64)
+    for b in range(batch):
+        ref = O.ssqueeze(Wx32[b], k[b], c32, nf)
+        got = Tx[b].cpu().numpy()
+        scale = np.abs(Wx32[b]).max() * c32.max() * np.sqrt(na)
+        assert np.abs(got - ref).max() <= 1e-6 * scale
