@@ -125,7 +125,8 @@ def main():
         "config": {"workload": "ssq_cwt, 1 signal, N=%d, GMW(3,60), nv=32 "
                                "log-piecewise (na=%d), reflect pad L=%d, "
                                "log-piecewise ssq_freqs" % (N, na, 2 * 2**int(np.round(np.log2(N)))),
-                   "batch": 1, "N": N, "na": na, "scales_per_path": paths},
+                   "batch": 1, "N": N, "na": na, "scales_per_path": paths,
+                   "plan": plan.describe()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

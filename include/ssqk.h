@@ -82,6 +82,10 @@ int ssqk_cwt_plan_destroy(ssqk_cwt_plan plan);
  * inverse FFTs, counts[1] overlap-save convolution, counts[2] full-length
  * four-step FFTs (no reference counterpart: plan introspection). */
 int ssqk_cwt_plan_paths(ssqk_cwt_plan plan, int64_t* counts);
+/* Text summary of the plan's scale classes (FFT sizes, scale counts,
+ * workgroup blocks), NUL-terminated and truncated to len bytes (no
+ * reference counterpart: plan introspection). */
+int ssqk_cwt_plan_describe(ssqk_cwt_plan plan, char* buf, size_t len);
 /* Device scratch bytes ssqk_cwt_exec needs for `batch` signals. */
 int ssqk_cwt_workspace_size(ssqk_cwt_plan plan, int64_t batch, size_t* bytes);
 

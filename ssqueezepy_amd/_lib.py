@@ -40,6 +40,7 @@ _SIGS = {
                                      c_int64, c_double, c_double]),
     'ssqk_cwt_plan_destroy': (c_int, [c_void_p]),
     'ssqk_cwt_plan_paths': (c_int, [c_void_p, ctypes.POINTER(c_int64)]),
+    'ssqk_cwt_plan_describe': (c_int, [c_void_p, ctypes.c_char_p, c_size_t]),
     'ssqk_cwt_workspace_size': (c_int, [c_void_p, c_int64,
                                         ctypes.POINTER(c_size_t)]),
     'ssqk_cwt_exec': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
@@ -145,6 +146,12 @@ class CwtPlan:
         c = (c_int64 * 3)()
         check(load().ssqk_cwt_plan_paths(self._h, c))
         return tuple(c)
+
+    def describe(self):
+        """Scale classes of the plan (FFT sizes, scale counts, blocks)."""
+        buf = ctypes.create_string_buffer(4096)
+        check(load().ssqk_cwt_plan_describe(self._h, buf, len(buf)))
+        return buf.value.decode()
 
     def workspace_bytes(self, batch):
         n = c_size_t()

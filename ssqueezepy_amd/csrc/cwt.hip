@@ -21,8 +21,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 using namespace ssqk;
@@ -775,6 +778,7 @@ struct ssqk_cwt_plan_s {
   struct Cls {
     int M = 0;
     int ntasks = 0;
+    int nscales = 0;
     int4* d_tasks = nullptr;
   };
   std::vector<Cls> classes;
@@ -793,6 +797,8 @@ struct ssqk_cwt_plan_s {
   struct OsCls {
     int F = 0;
     int ntasks = 0;
+    int nscales = 0;
+    int Hmin = 0, Hmax = 0;
     OsGroup* d_groups = nullptr;
     int2* d_tasks = nullptr;
   };
@@ -802,6 +808,12 @@ struct ssqk_cwt_plan_s {
   int n_os = 0;
   double center = 1.0;        // peak radian frequency of the wavelet
   int path_count[3] = {0, 0, 0};   // band, overlap-save, four-step
+  // side streams: the scale classes are spread over the caller's stream and
+  // these (fork after the forward FFT / at entry, join before returning)
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t ev_start = nullptr, ev_fwd = nullptr, ev_done[kSide] = {};
+  std::mutex exec_mu;
 };
 
 namespace {
@@ -923,6 +935,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
   std::vector<double> scales_h(scales, scales + na);
   std::vector<int2> band(na);
   std::vector<std::vector<int4>> cls_tasks(14);
+  std::vector<int> cls_scales(14, 0);
   std::vector<int> os_F(na, 0), os_H(na, 0);
   for (int64_t i = 0; i < na; ++i) {
     const double s = scales[i];
@@ -965,6 +978,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       const int rows = band_rows((int)M);
       for (int64_t a0 = 0; a0 < D; a0 += rows)
         cls_tasks[lM].push_back(make_int4((int)i, (int)a0, (int)ka, ilog2(D)));
+      cls_scales[lM]++;
       p->path_count[0]++;
     } else {
       p->full.push_back((int)i);
@@ -980,6 +994,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
     ssqk_cwt_plan_s::Cls c;
     c.M = 1 << lM;
     c.ntasks = (int)cls_tasks[lM].size();
+    c.nscales = cls_scales[lM];
     e = upload(&c.d_tasks, cls_tasks[lM]);
     p->classes.push_back(c);
   }
@@ -1040,8 +1055,12 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       ssqk_cwt_plan_s::OsCls oc;
       oc.F = Fs[c];
       std::vector<int2> tasks, pt;
+      oc.Hmin = 1 << 30;
       for (size_t gi = 0; gi < grp_by_F[c].size(); ++gi) {
         const OsGroup& g = grp_by_F[c][gi];
+        oc.nscales += g.ns;
+        oc.Hmin = std::min(oc.Hmin, g.H);
+        oc.Hmax = std::max(oc.Hmax, g.H);
         const int64_t nblk = (N + g.B - 1) / g.B;
         for (int64_t bk = 0; bk < nblk; ++bk) tasks.push_back(make_int2((int)gi, (int)bk));
         for (int r = 0; r < g.ns; ++r)
@@ -1062,6 +1081,12 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       e = hipMalloc(&p->d_coef, sizeof(float2) * (size_t)p->n_os * p->coef_stride);
     }
   }
+  for (int k = 0; k < ssqk_cwt_plan_s::kSide && e == hipSuccess; ++k) {
+    e = hipStreamCreateWithFlags(&p->side[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_done[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_start, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fwd, hipEventDisableTiming);
   if (e != hipSuccess) {
     ssqk_cwt_plan_destroy(p);
     return ssqk_host::check_hip(e, "plan upload");
@@ -1147,6 +1172,12 @@ extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
     (void)hipFree(c.d_tasks);
   }
   (void)hipFree(p->d_coef);
+  for (int k = 0; k < ssqk_cwt_plan_s::kSide; ++k) {
+    if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
+    if (p->ev_done[k]) (void)hipEventDestroy(p->ev_done[k]);
+  }
+  if (p->ev_start) (void)hipEventDestroy(p->ev_start);
+  if (p->ev_fwd) (void)hipEventDestroy(p->ev_fwd);
   delete p;
   return SSQK_OK;
 }
@@ -1157,6 +1188,32 @@ extern "C" int ssqk_cwt_plan_paths(ssqk_cwt_plan p, int64_t* counts) {
     return SSQK_ERR_ARG;
   }
   for (int k = 0; k < 3; ++k) counts[k] = p->path_count[k];
+  return SSQK_OK;
+}
+
+extern "C" int ssqk_cwt_plan_describe(ssqk_cwt_plan p, char* buf, size_t len) {
+  if (!p || !buf || len == 0) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_plan_describe");
+    return SSQK_ERR_ARG;
+  }
+  std::string d;
+  char t[160];
+  for (const auto& c : p->classes) {
+    std::snprintf(t, sizeof t, "band M=%d scales=%d blocks=%d; ", c.M, c.nscales, c.ntasks);
+    d += t;
+  }
+  for (const auto& o : p->os) {
+    std::snprintf(t, sizeof t, "overlap-save F=%d scales=%d H=%d..%d blocks=%d; ", o.F,
+                  o.nscales, o.Hmin, o.Hmax, o.ntasks);
+    d += t;
+  }
+  if (!p->full.empty()) {
+    std::snprintf(t, sizeof t, "four-step L=%dx%d scales=%d chunks=%d; ", p->L1, p->L2,
+                  (int)p->full.size(), (int)p->chunks.size());
+    d += t;
+  }
+  if (d.size() >= 2) d.resize(d.size() - 2);
+  std::snprintf(buf, len, "%s", d.c_str());
   return SSQK_OK;
 }
 
@@ -1197,8 +1254,59 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     ssqk_host::set_error("could not create the phase table");
     return SSQK_ERR_HIP;
   }
+  // work items, each on one stream: overlap-save classes (read x), band
+  // classes and the four-step chunks (read xh; the chunks share scratch so
+  // they stay together); spread by estimated cost over the caller's stream
+  // and the side streams, longest first
+  struct Item {
+    int kind, idx;
+    double cost;
+  };
+  std::vector<Item> items;
+  const double N = (double)p->N, L = (double)p->L;
+  for (size_t c = 0; c < p->os.size(); ++c) {
+    const auto& o = p->os[c];
+    items.push_back({0, (int)c, o.nscales * N * (std::log2((double)o.F) + 3) * o.F /
+                                    (double)std::max(o.F - 2 * o.Hmax, 1)});
+  }
+  for (size_t c = 0; c < p->classes.size(); ++c) {
+    const auto& k = p->classes[c];
+    items.push_back({1, (int)c, k.nscales * L * (std::log2((double)k.M) + 3)});
+  }
+  if (!p->chunks.empty())
+    items.push_back({2, 0, (double)p->full.size() * L * (std::log2(L) + 8)});
+  std::sort(items.begin(), items.end(),
+            [](const Item& u, const Item& v) { return u.cost > v.cost; });
+  constexpr int NS = ssqk_cwt_plan_s::kSide + 1;
+  hipStream_t streams[NS] = {st, p->side[0], p->side[1], p->side[2]};
+  double load[NS] = {0, 0, 0, 0};
+  std::vector<int> where(items.size());
+  for (size_t t = 0; t < items.size(); ++t) {
+    int best = 0;
+    for (int k = 1; k < NS; ++k)
+      if (load[k] < load[best]) best = k;
+    where[t] = best;
+    load[best] += items[t].cost;
+  }
+  bool used[NS] = {true, false, false, false};
+  for (size_t t = 0; t < items.size(); ++t) used[where[t]] = true;
+
+  std::lock_guard<std::mutex> lock(p->exec_mu);
+  SSQK_CHECK(hipEventRecord(p->ev_start, st));
+  for (int k = 1; k < NS; ++k)
+    if (used[k]) SSQK_CHECK(hipStreamWaitEvent(streams[k], p->ev_start, 0));
   int rc;
-  if (!p->classes.empty() || !p->full.empty()) {
+  const bool need_xh = !p->classes.empty() || !p->full.empty();
+  // overlap-save first (independent of the forward FFT)
+  for (size_t t = 0; t < items.size(); ++t) {
+    if (items[t].kind != 0) continue;
+    const auto& oc = p->os[items[t].idx];
+    rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), streams[where[t]], a, oc.d_groups,
+                   oc.d_tasks, p->d_coef, p->coef_stride, x, p->N, p->pad);
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+  }
+  if (need_xh) {
     FwdArgs f;
     f.x = x; f.N = p->N; f.n1 = p->n1; f.L = (int)p->L; f.pad = p->pad;
     f.xh = xh; f.xh_stride = xh_stride(p);
@@ -1207,29 +1315,41 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     rc = launch_fwd(f, (int)batch, st, p->fL1, p->fL2, (float2*)(ws + off_fwd));
     if (rc) return rc;
     SSQK_CHECK(hipGetLastError());
-  }
-  for (const auto& oc : p->os) {
-    rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), st, a, oc.d_groups, oc.d_tasks,
-                   p->d_coef, p->coef_stride, x, p->N, p->pad);
-    if (rc) return rc;
-    SSQK_CHECK(hipGetLastError());
-  }
-  for (const auto& c : p->classes) {
-    rc = launch_band(c.M, dim3(c.ntasks, (unsigned)batch), st, a, c.d_tasks);
-    if (rc) return rc;
-    SSQK_CHECK(hipGetLastError());
+    SSQK_CHECK(hipEventRecord(p->ev_fwd, st));
+    bool waited[NS] = {true, false, false, false};
+    for (size_t t = 0; t < items.size(); ++t) {
+      const int k = where[t];
+      if (items[t].kind == 0 || waited[k]) continue;
+      SSQK_CHECK(hipStreamWaitEvent(streams[k], p->ev_fwd, 0));
+      waited[k] = true;
+    }
   }
   float2* sW = (float2*)(ws + off_full);
   float2* sD = sW + (size_t)p->L * batch * p->slots;
-  for (const auto& ch : p->chunks) {
-    rc = launch_fullA(p->L2, dim3(ch.nA, (unsigned)batch), st, a, ch.d_A, p->d_band, p->L1,
-                      (int)batch, sW, sD);
-    if (rc) return rc;
-    SSQK_CHECK(hipGetLastError());
-    rc = launch_fullB(p->L1, dim3(ch.nB, (unsigned)batch), st, a, ch.d_B, p->L2, (int)batch,
-                      sW, sD);
-    if (rc) return rc;
-    SSQK_CHECK(hipGetLastError());
+  for (size_t t = 0; t < items.size(); ++t) {
+    hipStream_t sk = streams[where[t]];
+    if (items[t].kind == 1) {
+      const auto& c = p->classes[items[t].idx];
+      rc = launch_band(c.M, dim3(c.ntasks, (unsigned)batch), sk, a, c.d_tasks);
+      if (rc) return rc;
+      SSQK_CHECK(hipGetLastError());
+    } else if (items[t].kind == 2) {
+      for (const auto& ch : p->chunks) {
+        rc = launch_fullA(p->L2, dim3(ch.nA, (unsigned)batch), sk, a, ch.d_A, p->d_band,
+                          p->L1, (int)batch, sW, sD);
+        if (rc) return rc;
+        SSQK_CHECK(hipGetLastError());
+        rc = launch_fullB(p->L1, dim3(ch.nB, (unsigned)batch), sk, a, ch.d_B, p->L2,
+                          (int)batch, sW, sD);
+        if (rc) return rc;
+        SSQK_CHECK(hipGetLastError());
+      }
+    }
+  }
+  for (int k = 1; k < NS; ++k) {
+    if (!used[k]) continue;
+    SSQK_CHECK(hipEventRecord(p->ev_done[k - 1], streams[k]));
+    SSQK_CHECK(hipStreamWaitEvent(st, p->ev_done[k - 1], 0));
   }
   return SSQK_OK;
 }
