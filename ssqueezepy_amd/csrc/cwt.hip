@@ -63,6 +63,8 @@ struct CwtArgs {
   BinDev bp;
   const float2* ptab;       // phase table of length L (ssqk::phase_tab)
   int pS;
+  const float* psih;        // psih(s_i xi_k) table, scale i at psih_off[i] + (k - ka_i)
+  const int64_t* psih_off;
 };
 
 // Source index of padded sample n (numpy pad modes, utils/common.py:123-158);
@@ -124,16 +126,39 @@ __device__ __forceinline__ void emit(const CwtArgs& a, int b, int i, int64_t n,
 // X = psih(s xi_k) v and Xd = X * i xi_k / dt for a positive bin k, where
 // v = xh[k] was loaded by the caller (loads are kept out of branches so that
 // they are issued together).  valid = false gives zeros.
-__device__ __forceinline__ void band_value(const CwtArgs& a, float2 v, double s, int k,
-                                           bool valid, float2& X, float2& Xd) {
-  const int half = a.L >> 1;
+// psih(s xi_k) rounded to float32 with the Nyquist bin halved
+// (wavelets.py:86-95); the plan tabulates it once per scale.
+__device__ __forceinline__ float psih_value(const WavDev& w, double s, int k, int L) {
+  const double xi = (double)k * (kTwoPi / (double)L);
+  double ps = psih_eval(w, s * xi);
+  if (k == (L >> 1)) ps *= 0.5;
+  return (float)ps;
+}
+
+__device__ __forceinline__ void band_value(const CwtArgs& a, float2 v, float pf, int k,
+                                           float2& X, float2& Xd) {
   const double xi = (double)k * (kTwoPi / (double)a.L);
-  double ps = valid ? psih_eval(a.wav, s * xi) : 0.0;
-  if (k == half) ps *= 0.5;                 // Nyquist halving (wavelets.py:86-95)
-  const float pf = (float)ps;
   X = make_float2(pf * v.x, pf * v.y);
   const float q = (float)(xi * a.inv_dt);
   Xd = make_float2(-q * X.y, q * X.x);
+}
+
+// psih table of the band and four-step scales: segment = {scale, first
+// bin, length, offset}; bins past the Nyquist bin are zero.
+struct PsihSeg {
+  int i, ka, len, pad;
+  int64_t off;
+};
+
+__global__ __launch_bounds__(256) void k_psih_tab(WavDev w, const double* __restrict__ scales,
+                                                  const PsihSeg* __restrict__ segs, int L,
+                                                  float* __restrict__ out) {
+  const PsihSeg g = segs[blockIdx.x];
+  const double s = scales[g.i];
+  for (int m = threadIdx.x; m < g.len; m += 256) {
+    const int k = g.ka + m;
+    out[g.off + m] = k <= (L >> 1) ? psih_value(w, s, k, L) : 0.f;
+  }
 }
 
 // ------------------------------------------------------------- band path
@@ -151,23 +176,25 @@ __global__ __launch_bounds__(band_threads(M)) void k_cwt_band(CwtArgs a,
   const int i = t.x, a0 = t.y, ka = t.z, D = 1 << t.w;
   const int L = a.L, half = L >> 1;
   const float2* xh = a.xh + (int64_t)b * a.xh_stride;
-  const double s = a.scales[i];
+  const float* ps = a.psih + a.psih_off[i];
 
-  // band samples into row 0
+  // band samples into row 0 (the table is zero past the Nyquist bin)
   constexpr int IM = (M + kNT - 1) / kNT;
   {
     float2 v[IM];
+    float pv[IM];
 #pragma unroll
     for (int it = 0; it < IM; ++it) {
       const int m = threadIdx.x + it * kNT;
       v[it] = xh[min(ka + m, half)];
+      pv[it] = ps[min(m, M - 1)];
     }
 #pragma unroll
     for (int it = 0; it < IM; ++it) {
       const int m = threadIdx.x + it * kNT;
       if (m < M) {
         float2 X, Xd;
-        band_value(a, v[it], s, ka + m, ka + m <= half, X, Xd);
+        band_value(a, v[it], pv[it], ka + m, X, Xd);
         sW[Lds<M>::idx(0, m)] = X;
         sD[Lds<M>::idx(0, m)] = Xd;
       }
@@ -249,16 +276,19 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullA(CwtArgs a, const int4* __
   const int i = t.x, k1_0 = t.y, slot = t.z;
   const int2 kab = band[i];
   const float2* xh = a.xh + (int64_t)b * a.xh_stride;
-  const double s = a.scales[i];
+  const float* ps = a.psih + a.psih_off[i];
   constexpr int IA = (G * L2 + kNT - 1) / kNT;
   {
     float2 v[IA];
+    float pv[IA];
 #pragma unroll
     for (int it = 0; it < IA; ++it) {
       const int e = threadIdx.x + it * kNT;
       const int g = e % G, k2 = e / G;
       const int k = k1_0 + g + L1 * k2;
-      v[it] = xh[min(max(k, kab.x), max(kab.y, kab.x))];
+      const int kc = min(max(k, kab.x), max(kab.y, kab.x));
+      v[it] = xh[kc];
+      pv[it] = ps[kc - kab.x];
     }
 #pragma unroll
     for (int it = 0; it < IA; ++it) {
@@ -268,7 +298,7 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullA(CwtArgs a, const int4* __
       if (e < G * L2) {
         float2 X = make_float2(0.f, 0.f), Xd = X;
         const bool in = k >= kab.x && k <= kab.y;
-        if (in) band_value(a, v[it], s, k, true, X, Xd);
+        if (in) band_value(a, v[it], pv[it], k, X, Xd);
         sW[Lds<L2>::idx(g, k2)] = X;
         sD[Lds<L2>::idx(g, k2)] = Xd;
       }
@@ -353,6 +383,7 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
                                                  const int2* __restrict__ tasks,
                                                  const float2* __restrict__ coef,
                                                  int coef_stride,
+                                                 const int2* __restrict__ coef_band,
                                                  const float2* __restrict__ tw,
                                                  const float* __restrict__ x, int64_t x_stride,
                                                  int pad) {
@@ -399,13 +430,18 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
   }
   __syncthreads();
   for (int s0 = 0; s0 < g.ns; s0 += SR) {
+    // kernel spectra are read on their band only (psih below 1e-10 of its
+    // peak elsewhere); lanes outside re-read the band edge and store zero
 #pragma unroll
     for (int it = 0; it < IC; ++it) {
       const int e = threadIdx.x + it * NT;
       const int r = e / F, k = e - r * F;
-      const float2* c = coef + (int64_t)(g.rank + min(s0 + r, g.ns - 1)) * coef_stride;
-      const float2 cA = c[k], cD = c[F + k];
-      const bool ok = s0 + r < g.ns;
+      const int rank = g.rank + min(s0 + r, g.ns - 1);
+      const int2 kb = coef_band[rank];
+      const float2* c = coef + (int64_t)rank * coef_stride;
+      const int kc = min(max(k, kb.x), kb.y);
+      const float2 cA = c[kc], cD = c[F + kc];
+      const bool ok = s0 + r < g.ns && k == kc;
       sA[Lds<F>::idx(r, k)] = ok ? cmul(Sreg[it], cA) : make_float2(0.f, 0.f);
       sD[Lds<F>::idx(r, k)] = ok ? cmul(Sreg[it], cD) : make_float2(0.f, 0.f);
     }
@@ -593,15 +629,14 @@ int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
 #define SSQK_OS_CASES(MACRO) MACRO(256) MACRO(512) MACRO(1024) MACRO(2048) MACRO(4096)
 
 int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup* groups,
-              const int2* tasks, const float2* coef, int coef_stride, const float* x,
-              int64_t x_stride, int pad) {
+              const int2* tasks, const float2* coef, int coef_stride, const int2* coef_band,
+              const float* x, int64_t x_stride, int pad) {
   SSQK_GET_TW(tw, F);
   switch (F) {
 #define C(m)                                                                        \
   case m:                                                                           \
     hipLaunchKernelGGL(k_cwt_os<m>, grid, dim3(OsShape<m>::NT), 0, st, a, groups,   \
-                       tasks, coef,                                                 \
-                       coef_stride, tw, x, x_stride, pad);                          \
+                       tasks, coef, coef_stride, coef_band, tw, x, x_stride, pad);  \
     break;
     SSQK_OS_CASES(C)
 #undef C
@@ -804,6 +839,9 @@ struct ssqk_cwt_plan_s {
   };
   std::vector<OsCls> os;
   float2* d_coef = nullptr;   // [rank][2][F], row stride coef_stride
+  int2* d_coef_band = nullptr;  // [rank] bins of the F-grid band
+  float* d_psih = nullptr;    // psih tables of band / four-step scales
+  int64_t* d_psih_off = nullptr;
   int coef_stride = 0;
   int n_os = 0;
   double center = 1.0;        // peak radian frequency of the wavelet
@@ -937,6 +975,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
   std::vector<std::vector<int4>> cls_tasks(14);
   std::vector<int> cls_scales(14, 0);
   std::vector<int> os_F(na, 0), os_H(na, 0);
+  std::vector<int64_t> tab_len(na, 0);
   for (int64_t i = 0; i < na; ++i) {
     const double s = scales[i];
     if (!(s > 0)) {
@@ -979,9 +1018,11 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       for (int64_t a0 = 0; a0 < D; a0 += rows)
         cls_tasks[lM].push_back(make_int4((int)i, (int)a0, (int)ka, ilog2(D)));
       cls_scales[lM]++;
+      tab_len[i] = M;
       p->path_count[0]++;
     } else {
       p->full.push_back((int)i);
+      tab_len[i] = K;
       p->path_count[2]++;
     }
   }
@@ -1021,8 +1062,42 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
   }
   if (L > kSmallFwd) split_four_step(L, &p->fL1, &p->fL2);
 
+  // psih tables (float32, computed once here in float64)
+  if (e == hipSuccess) {
+    std::vector<PsihSeg> segs;
+    std::vector<int64_t> off(na, 0);
+    int64_t total = 0;
+    for (int64_t i = 0; i < na; ++i) {
+      off[i] = total;
+      if (!tab_len[i]) continue;
+      for (int64_t m0 = 0; m0 < tab_len[i]; m0 += 8192) {
+        PsihSeg g{};
+        g.i = (int)i;
+        g.ka = band[i].x + (int)m0;
+        g.len = (int)std::min<int64_t>(8192, tab_len[i] - m0);
+        g.off = total + m0;
+        segs.push_back(g);
+      }
+      total += tab_len[i];
+    }
+    e = upload(&p->d_psih_off, off);
+    if (e == hipSuccess) e = hipMalloc(&p->d_psih, sizeof(float) * std::max<int64_t>(total, 1));
+    if (e == hipSuccess && !segs.empty()) {
+      PsihSeg* d_segs = nullptr;
+      e = upload(&d_segs, segs);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_psih_tab, dim3((unsigned)segs.size()), dim3(256), 0, nullptr,
+                           p->wav, p->d_scales, d_segs, (int)L, p->d_psih);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      (void)hipFree(d_segs);
+    }
+  }
+
   // overlap-save groups: runs of consecutive scales sharing F, up to 8
   std::vector<int> os_scales;
+  std::vector<int2> coef_band;
   std::vector<OsGroup> all_groups;
   std::vector<std::vector<int2>> prep_tasks;
   int Hmax = 0;
@@ -1039,6 +1114,11 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       g.rank = (int)os_scales.size();
       while (i < na && os_F[i] == F && g.ns < 8) {
         g.H = std::max(g.H, os_H[i]);
+        // F-grid band of psih(s xi) above band_rel, two bins of margin
+        const double d = 2 * M_PI / (double)F * scales[i];
+        const int ka = (int)std::max(0.0, std::ceil(wlo / d) - 2);
+        const int kb = (int)std::min((double)F / 2, std::floor(whi / d) + 2);
+        coef_band.push_back(make_int2(std::min(ka, kb), kb));
         os_scales.push_back((int)i);
         ++g.ns;
         ++i;
@@ -1079,6 +1159,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       for (const auto& oc : p->os) Fmax = std::max(Fmax, oc.F);
       p->coef_stride = 2 * Fmax;
       e = hipMalloc(&p->d_coef, sizeof(float2) * (size_t)p->n_os * p->coef_stride);
+      if (e == hipSuccess) e = upload(&p->d_coef_band, coef_band);
     }
   }
   for (int k = 0; k < ssqk_cwt_plan_s::kSide && e == hipSuccess; ++k) {
@@ -1172,6 +1253,9 @@ extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
     (void)hipFree(c.d_tasks);
   }
   (void)hipFree(p->d_coef);
+  (void)hipFree(p->d_psih);
+  (void)hipFree(p->d_psih_off);
+  (void)hipFree(p->d_coef_band);
   for (int k = 0; k < ssqk_cwt_plan_s::kSide; ++k) {
     if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
     if (p->ev_done[k]) (void)hipEventDestroy(p->ev_done[k]);
@@ -1250,6 +1334,8 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
   a.bp = make_bin_dev(bp ? bp : &zero);
   a.ptab = ssqk_host::phase_table((int)p->L);
   a.pS = phase_tab_split(p->L);
+  a.psih = p->d_psih;
+  a.psih_off = p->d_psih_off;
   if (!a.ptab) {
     ssqk_host::set_error("could not create the phase table");
     return SSQK_ERR_HIP;
@@ -1302,7 +1388,7 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     if (items[t].kind != 0) continue;
     const auto& oc = p->os[items[t].idx];
     rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), streams[where[t]], a, oc.d_groups,
-                   oc.d_tasks, p->d_coef, p->coef_stride, x, p->N, p->pad);
+                   oc.d_tasks, p->d_coef, p->coef_stride, p->d_coef_band, x, p->N, p->pad);
     if (rc) return rc;
     SSQK_CHECK(hipGetLastError());
   }
