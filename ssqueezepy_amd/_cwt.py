@@ -9,7 +9,7 @@ import numpy as np
 from . import _lib
 from ._common import WARN, _process_fs_and_t, p2up
 from ._scales import process_scales
-from ._tensors import signal_to_device, to_output
+from ._tensors import device_constant, signal_to_device, to_output
 from .wavelets import Wavelet, KIND_GMW
 
 __all__ = ['cwt']
@@ -94,7 +94,7 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
     Wx = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
     dWx = torch.empty_like(Wx) if derivative else None
     run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=dWx)
-    scales_out = torch.as_tensor(scales64.astype(wavelet.dtype),
-                                 device=x_dev.device)
+    scales_out = device_constant(scales64.astype(wavelet.dtype), torch.float32,
+                                 x_dev.device, copy=True)
     Wx, scales_out, dWx = [to_output(g, astensor) for g in (Wx, scales_out, dWx)]
     return (Wx, scales_out, dWx) if derivative else (Wx, scales_out)

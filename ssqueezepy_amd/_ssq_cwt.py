@@ -11,7 +11,7 @@ from . import _lib
 from ._common import EPS32, EPS64, _process_fs_and_t
 from ._cwt import _nan_checks, run_cwt
 from ._scales import process_scales
-from ._tensors import as_device_tensor, signal_to_device, to_output
+from ._tensors import as_device_tensor, device_constant, signal_to_device, to_output
 from .ssqueezing import (_apply_squeezing, _check_ssqueezing_args,
                          bin_params_from_freqs, compute_associated_frequencies,
                          cwt_const)
@@ -119,12 +119,7 @@ def _ssq_cwt_grid(N, wavelet, scales='log-piecewise', nv=None, fs=None, t=None,
 def _device_const(g, dev):
     """float32 squeeze weights on ``dev``, uploaded once per grid."""
     import torch
-    cache = g.setdefault('_const_dev', {})
-    t = cache.get(str(dev))
-    if t is None:
-        t = torch.as_tensor(g['const'], dtype=torch.float32, device=dev)
-        cache[str(dev)] = t
-    return t
+    return device_constant(g['const'], torch.float32, dev)
 
 
 def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
@@ -171,7 +166,7 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
     _lib.squeeze(Wsq, bins, _device_const(g, dev), na, Tx)
 
     ssq_freqs = g['ssq_freqs'][::-1]          # ssqueezing.py:216-222
-    scales_out = torch.as_tensor(g['scales'], device=dev, dtype=torch.float32) \
+    scales_out = device_constant(g['scales'], torch.float32, dev, copy=True) \
         if astensor else g['scales'].copy()
     Tx, Wx, w, dWx, scales_out = [to_output(v, astensor)
                                   for v in (Tx, Wx, w, dWx, scales_out)]

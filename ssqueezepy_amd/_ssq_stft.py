@@ -11,7 +11,7 @@ from . import _lib
 from ._common import EPS32, _process_fs_and_t
 from ._scales import infer_scaletype
 from ._stft import stft_setup
-from ._tensors import as_device_tensor, signal_to_device, to_output
+from ._tensors import as_device_tensor, device_constant, signal_to_device, to_output
 from .ssqueezing import _apply_squeezing, _check_ssqueezing_args, bin_params_from_freqs
 
 __all__ = ['ssq_stft', 'phase_stft']
@@ -56,9 +56,9 @@ def ssq_stft(x, window=None, n_fft=None, win_len=None, hop_len=1, fs=None, t=Non
     Sx = torch.empty(shape, dtype=torch.complex64, device=dev)
     dSx = torch.empty_like(Sx)
     bins = torch.empty(shape, dtype=torch.int16, device=dev)
-    win = torch.as_tensor(window, device=dev)
-    dwin = torch.as_tensor(diff_window, device=dev)
-    Sfs_t = torch.as_tensor(Sfs, dtype=torch.float32, device=dev)
+    win = device_constant(window, torch.float32, dev)
+    dwin = device_constant(diff_window, torch.float32, dev)
+    Sfs_t = device_constant(Sfs, torch.float32, dev)
     w = None
     if get_w:
         _lib.stft_exec(x_dev, n_fft, hop_len, padtype, modulated, win, dwin, fs,

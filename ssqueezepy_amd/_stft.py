@@ -9,7 +9,7 @@ import scipy.signal as sig
 
 from . import _lib
 from ._common import WARN, _process_fs_and_t
-from ._tensors import signal_to_device, to_output
+from ._tensors import device_constant, signal_to_device, to_output
 from .configs import gdefaults
 from .wavelets import _xifn
 
@@ -112,8 +112,8 @@ def stft(x, window=None, n_fft=None, win_len=None, hop_len=1, fs=None, t=None,
     shape = (rows, n_frames) if x.ndim == 1 else (x.shape[0], rows, n_frames)
     Sx = torch.empty(shape, dtype=torch.complex64, device=dev)
     dSx = torch.empty_like(Sx) if derivative else None
-    win = torch.as_tensor(window, device=dev)
-    dwin = torch.as_tensor(diff_window, device=dev) if derivative else None
+    win = device_constant(window, torch.float32, dev)
+    dwin = device_constant(diff_window, torch.float32, dev) if derivative else None
     _lib.stft_exec(x_dev, n_fft, hop_len, padtype, modulated, win, dwin, fs, Sx,
                    dSx)
     return (Sx, dSx) if derivative else Sx

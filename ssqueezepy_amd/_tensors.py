@@ -5,7 +5,13 @@ on the current CUDA (HIP) device and return torch tensors when
 ``astensor=True``, NumPy arrays otherwise, as the reference does under
 SSQ_GPU=1 (ssqueezepy/utils/backend.py:26-42, _ssq_cwt.py:297-300).
 """
+from collections import OrderedDict
+import threading
+
 import numpy as np
+
+_consts = OrderedDict()
+_consts_lock = threading.Lock()
 
 
 def _torch():
@@ -73,3 +79,28 @@ def to_output(t, astensor):
     if isinstance(t, torch.Tensor):
         return t.cpu().numpy()
     return t
+
+
+def device_constant(arr, dtype, device, copy=False):
+    """Small host array ``arr`` as a ``dtype`` tensor on ``device``.
+
+    A pageable host->device copy makes the host wait for the stream, which
+    keeps it from queueing the next transform while the GPU works; these
+    arrays (scales, squeeze weights, windows, frequency grids) repeat from
+    call to call, so each is uploaded once and kept (LRU of 64).  With
+    ``copy`` the caller gets its own device copy (an asynchronous
+    device-to-device copy), for arrays handed back to the user."""
+    torch = _torch()
+    a = np.ascontiguousarray(np.asarray(arr))
+    key = (str(device), str(dtype), a.dtype.str, a.shape, a.tobytes())
+    with _consts_lock:
+        t = _consts.get(key)
+        if t is not None:
+            _consts.move_to_end(key)
+    if t is None:
+        t = torch.as_tensor(a, device=device).to(dtype)
+        with _consts_lock:
+            _consts[key] = t
+            while len(_consts) > 64:
+                _consts.popitem(last=False)
+    return t.clone() if copy else t

@@ -4,7 +4,7 @@
 import numpy as np
 
 from . import _lib
-from ._tensors import as_device_tensor
+from ._tensors import as_device_tensor, device_constant
 from .ssqueezing import bin_params_from_freqs
 
 __all__ = ['ssqueeze_fast', 'indexed_sum_onfly', 'phase_cwt_gpu',
@@ -17,7 +17,7 @@ def _const_tensor(const, na, like):
                    dtype=np.float64).ravel()
     if c.size != na:
         c = np.full(na, float(c[0]) if c.size else 1.)
-    return torch.as_tensor(c, dtype=torch.float32, device=like.device)
+    return device_constant(c, torch.float32, like.device)
 
 
 def ssqueeze_fast(Wx, dWx, ssq_freqs, const, logscale=False, flipud=False,

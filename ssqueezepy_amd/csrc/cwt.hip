@@ -164,15 +164,11 @@ __global__ __launch_bounds__(256) void k_psih_tab(WavDev w, const double* __rest
 // ------------------------------------------------------------- band path
 // task = {scale i, first phase a0, band start ka, log2 D}
 template <int M>
-__global__ __launch_bounds__(band_threads(M)) void k_cwt_band(CwtArgs a,
-                                                             const int4* __restrict__ tasks,
-                                                             const float2* __restrict__ tw) {
+__device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const int b,
+                                          const float2* __restrict__ tw,
+                                          float2* __restrict__ sW, float2* __restrict__ sD) {
   constexpr int ROWS = band_rows(M);
   constexpr int kNT = band_threads(M);
-  __shared__ float2 sW[Lds<M>::elems(ROWS)];
-  __shared__ float2 sD[Lds<M>::elems(ROWS)];
-  const int4 t = tasks[blockIdx.x];
-  const int b = blockIdx.y;
   const int i = t.x, a0 = t.y, ka = t.z, D = 1 << t.w;
   const int L = a.L, half = L >> 1;
   const float2* xh = a.xh + (int64_t)b * a.xh_stride;
@@ -253,6 +249,15 @@ __global__ __launch_bounds__(band_threads(M)) void k_cwt_band(CwtArgs a,
       emit(a, b, i, n, cmul(sW[Lds<M>::idx(r, q)], p), cmul(sD[Lds<M>::idx(r, q)], p));
     }
   }
+}
+
+template <int M>
+__global__ __launch_bounds__(band_threads(M)) void k_cwt_band(CwtArgs a,
+                                                             const int4* __restrict__ tasks,
+                                                             const float2* __restrict__ tw) {
+  __shared__ float2 sW[Lds<M>::elems(band_rows(M))];
+  __shared__ float2 sD[Lds<M>::elems(band_rows(M))];
+  band_task<M>(a, tasks[blockIdx.x], blockIdx.y, tw, sW, sD);
 }
 
 // ------------------------------------------------------------- full path
@@ -430,11 +435,15 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
   }
   __syncthreads();
   for (int s0 = 0; s0 < g.ns; s0 += SR) {
+    // opaque thread index per scale batch: the compiler would otherwise keep
+    // this loop's LDS addresses live across it (registers, occupancy)
+    int tx = threadIdx.x;
+    asm volatile("" : "+v"(tx));
     // kernel spectra are read on their band only (psih below 1e-10 of its
     // peak elsewhere); lanes outside re-read the band edge and store zero
 #pragma unroll
     for (int it = 0; it < IC; ++it) {
-      const int e = threadIdx.x + it * NT;
+      const int e = tx + it * NT;
       const int r = e / F, k = e - r * F;
       const int rank = g.rank + min(s0 + r, g.ns - 1);
       const int2 kb = coef_band[rank];
@@ -447,7 +456,7 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
     }
     float2* const bufs[2] = {sA, sD};
     lds_fft<F, SR, +1, NT, 2>(bufs, tw);
-    for (int e = threadIdx.x; e < SR * g.B; e += NT) {
+    for (int e = tx; e < SR * g.B; e += NT) {
       const int r = e / g.B, tt = e - r * g.B;
       if (s0 + r >= g.ns) continue;
       const int64_t j = j_base + tt;
@@ -849,6 +858,7 @@ struct ssqk_cwt_plan_s {
   // side streams: the scale classes are spread over the caller's stream and
   // these (fork after the forward FFT / at entry, join before returning)
   static constexpr int kSide = 3;
+  int nside = kSide;          // side streams in use (SSQK_SIDE_STREAMS, 0..3)
   hipStream_t side[kSide] = {};
   hipEvent_t ev_start = nullptr, ev_fwd = nullptr, ev_done[kSide] = {};
   std::mutex exec_mu;
@@ -1162,6 +1172,8 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       if (e == hipSuccess) e = upload(&p->d_coef_band, coef_band);
     }
   }
+  if (const char* ev = std::getenv("SSQK_SIDE_STREAMS"))
+    p->nside = std::max(0, std::min(ssqk_cwt_plan_s::kSide, std::atoi(ev)));
   for (int k = 0; k < ssqk_cwt_plan_s::kSide && e == hipSuccess; ++k) {
     e = hipStreamCreateWithFlags(&p->side[k], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_done[k], hipEventDisableTiming);
@@ -1369,7 +1381,7 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
   std::vector<int> where(items.size());
   for (size_t t = 0; t < items.size(); ++t) {
     int best = 0;
-    for (int k = 1; k < NS; ++k)
+    for (int k = 1; k <= p->nside; ++k)
       if (load[k] < load[best]) best = k;
     where[t] = best;
     load[best] += items[t].cost;
@@ -1383,15 +1395,8 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     if (used[k]) SSQK_CHECK(hipStreamWaitEvent(streams[k], p->ev_start, 0));
   int rc;
   const bool need_xh = !p->classes.empty() || !p->full.empty();
-  // overlap-save first (independent of the forward FFT)
-  for (size_t t = 0; t < items.size(); ++t) {
-    if (items[t].kind != 0) continue;
-    const auto& oc = p->os[items[t].idx];
-    rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), streams[where[t]], a, oc.d_groups,
-                   oc.d_tasks, p->d_coef, p->coef_stride, p->d_coef_band, x, p->N, p->pad);
-    if (rc) return rc;
-    SSQK_CHECK(hipGetLastError());
-  }
+  // the forward FFT leads the caller's stream; overlap-save items (which
+  // read x, not xh) go first on the side streams, band items wait for xh
   if (need_xh) {
     FwdArgs f;
     f.x = x; f.N = p->N; f.n1 = p->n1; f.L = (int)p->L; f.pad = p->pad;
@@ -1402,6 +1407,16 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     if (rc) return rc;
     SSQK_CHECK(hipGetLastError());
     SSQK_CHECK(hipEventRecord(p->ev_fwd, st));
+  }
+  for (size_t t = 0; t < items.size(); ++t) {
+    if (items[t].kind != 0) continue;
+    const auto& oc = p->os[items[t].idx];
+    rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), streams[where[t]], a, oc.d_groups,
+                   oc.d_tasks, p->d_coef, p->coef_stride, p->d_coef_band, x, p->N, p->pad);
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+  }
+  if (need_xh) {
     bool waited[NS] = {true, false, false, false};
     for (size_t t = 0; t < items.size(); ++t) {
       const int k = where[t];

@@ -211,7 +211,11 @@ __device__ __forceinline__ void stockham_pass(float2* const (&buf)[NCH],
   constexpr int NIT = NCH * NBF;
   constexpr int ITER = (NIT + NT - 1) / NT;
   float2 v[ITER][R];
-  const int tid = threadIdx.x;
+  // opaque copy of the thread index: keeps the compiler from hoisting each
+  // pass's LDS addresses out of callers' loops, where dozens of them would
+  // stay live in registers and cut occupancy
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
     const int wi = tid + it * NT;

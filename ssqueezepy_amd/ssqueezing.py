@@ -14,7 +14,7 @@ from . import _lib
 from ._common import (EPS64, NOTE, WARN, _process_fs_and_t, assert_is_one_of,
                       p2up, pi)
 from ._scales import infer_scaletype, logscale_transition_idx, process_scales
-from ._tensors import as_device_tensor, to_output
+from ._tensors import as_device_tensor, device_constant, to_output
 from .wavelets import center_frequency
 
 __all__ = ['ssqueeze', 'bin_params_from_freqs', 'compute_associated_frequencies']
@@ -242,15 +242,15 @@ def ssqueeze(Wx, w=None, ssq_freqs=None, scales=None, Sfs=None, fs=None, t=None,
     if w is None:
         Sfs_t = None
         if transform == 'stft':
-            Sfs_t = torch.as_tensor(np.asarray(
+            Sfs_t = device_constant(np.asarray(
                 Sfs.detach().cpu() if hasattr(Sfs, 'detach') else Sfs),
-                dtype=torch.float32, device=device)
+                torch.float32, device)
         _lib.bins_from_dwx(Wx, dWx, Sfs_t, bp, gamma, bins)
     else:
         _lib.bins_from_w(w, bp, bins)
     Wsq = _apply_squeezing(Wx, squeezing).contiguous()
     Tx = torch.empty_like(Wx)
-    cst = torch.as_tensor(const, dtype=torch.float32, device=device)
+    cst = device_constant(const, torch.float32, device)
     _lib.squeeze(Wsq, bins, cst, na, Tx)
 
     if (transform == 'cwt' and not flipud) or flipud:
