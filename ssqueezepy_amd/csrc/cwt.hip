@@ -67,6 +67,21 @@ struct CwtArgs {
   const int64_t* psih_off;
 };
 
+// XCD-aware task order.  Workgroups are dispatched to the 8 XCDs round
+// robin (block b to XCD b % 8), each XCD with its own L2.  Block b takes
+// task j = b / 8 of XCD x's contiguous share of the task list, so
+// consecutive tasks (the phases of one scale, the segments of one
+// overlap-save group) meet in one L2: their strided output writes merge
+// into whole lines there and shared inputs are fetched once per XCD.
+// Launch with xcd_grid(ntasks) blocks; -1 = no task for this block.
+__device__ __forceinline__ int xcd_task(int b, int ntasks) {
+  const int x = b & 7, j = b >> 3;
+  const int q = ntasks >> 3, r = ntasks & 7;
+  if (j >= q + (x < r ? 1 : 0)) return -1;
+  return x * q + (x < r ? x : r) + j;
+}
+inline unsigned xcd_grid(unsigned ntasks) { return (ntasks + 7) / 8 * 8; }
+
 // Source index of padded sample n (numpy pad modes, utils/common.py:123-158);
 // -1 for a zero sample.
 __device__ __forceinline__ int64_t pad_src(int64_t n, int64_t N, int64_t n1, int type) {
@@ -254,10 +269,13 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
 template <int M>
 __global__ __launch_bounds__(band_threads(M)) void k_cwt_band(CwtArgs a,
                                                              const int4* __restrict__ tasks,
+                                                             int ntasks,
                                                              const float2* __restrict__ tw) {
   __shared__ float2 sW[Lds<M>::elems(band_rows(M))];
   __shared__ float2 sD[Lds<M>::elems(band_rows(M))];
-  band_task<M>(a, tasks[blockIdx.x], blockIdx.y, tw, sW, sD);
+  const int task = xcd_task(blockIdx.x, ntasks);
+  if (task < 0) return;
+  band_task<M>(a, tasks[task], blockIdx.y, tw, sW, sD);
 }
 
 // ------------------------------------------------------------- full path
@@ -268,7 +286,7 @@ __global__ __launch_bounds__(band_threads(M)) void k_cwt_band(CwtArgs a,
 // taskA = {i, k1_0, slot, -}, taskB = {i, n2_0, slot, -}; band[i] = {ka, kb}.
 template <int L2>
 __global__ __launch_bounds__(kPF / 8) void k_cwt_fullA(CwtArgs a, const int4* __restrict__ tasks,
-                                                   const int2* __restrict__ band,
+                                                   int ntasks, const int2* __restrict__ band,
                                                    const float2* __restrict__ tw, int L1,
                                                    int batch, float2* __restrict__ scrW,
                                                    float2* __restrict__ scrD) {
@@ -276,7 +294,9 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullA(CwtArgs a, const int4* __
   constexpr int kNT = kPF / 8;
   __shared__ float2 sW[Lds<L2>::elems(G)];
   __shared__ float2 sD[Lds<L2>::elems(G)];
-  const int4 t = tasks[blockIdx.x];
+  const int task = xcd_task(blockIdx.x, ntasks);
+  if (task < 0) return;
+  const int4 t = tasks[task];
   const int b = blockIdx.y;
   const int i = t.x, k1_0 = t.y, slot = t.z;
   const int2 kab = band[i];
@@ -333,6 +353,7 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullA(CwtArgs a, const int4* __
 
 template <int L1>
 __global__ __launch_bounds__(kPF / 8) void k_cwt_fullB(CwtArgs a, const int4* __restrict__ tasks,
+                                                   int ntasks,
                                                    const float2* __restrict__ tw, int L2,
                                                    int batch, const float2* __restrict__ scrW,
                                                    const float2* __restrict__ scrD) {
@@ -340,7 +361,9 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullB(CwtArgs a, const int4* __
   constexpr int kNT = kPF / 8;
   __shared__ float2 sW[Lds<L1>::elems(G)];
   __shared__ float2 sD[Lds<L1>::elems(G)];
-  const int4 t = tasks[blockIdx.x];
+  const int task = xcd_task(blockIdx.x, ntasks);
+  if (task < 0) return;
+  const int4 t = tasks[task];
   const int b = blockIdx.y;
   const int i = t.x, n2_0 = t.y, slot = t.z;
   const int64_t base = ((int64_t)slot * batch + b) * (int64_t)L2;
@@ -386,6 +409,7 @@ struct OsShape {
 template <int F>
 __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGroup* __restrict__ groups,
                                                  const int2* __restrict__ tasks,
+                                                 int ntasks,
                                                  const float2* __restrict__ coef,
                                                  int coef_stride,
                                                  const int2* __restrict__ coef_band,
@@ -398,7 +422,9 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
   // registers, in the layout of the multiply stage
   __shared__ float2 sA[Lds<F>::elems(SR)];
   __shared__ float2 sD[Lds<F>::elems(SR)];
-  const int2 t = tasks[blockIdx.x];
+  const int task = xcd_task(blockIdx.x, ntasks);
+  if (task < 0) return;
+  const int2 t = tasks[task];
   const OsGroup g = groups[t.x];
   const int b = blockIdx.y;
   const int64_t L = a.L;
@@ -584,11 +610,12 @@ __global__ __launch_bounds__(kNT) void k_fwd_B(FwdArgs f, const float2* __restri
 
 int launch_band(int M, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks) {
   SSQK_GET_TW(tw, M);
+  const dim3 g8(xcd_grid(grid.x), grid.y);
   switch (M) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_cwt_band<m>, grid, dim3(band_threads(m)), 0, st, a, tasks, \
-                       tw);                                    \
+    hipLaunchKernelGGL(k_cwt_band<m>, g8, dim3(band_threads(m)), 0, st, a, tasks,  \
+                       (int)grid.x, tw);                                            \
     break;
     SSQK_POW2_CASES(C)
 #undef C
@@ -602,11 +629,12 @@ int launch_band(int M, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* 
 int launch_fullA(int L2, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks,
                  const int2* band, int L1, int batch, float2* sW, float2* sD) {
   SSQK_GET_TW(tw, L2);
+  const dim3 g8(xcd_grid(grid.x), grid.y);
   switch (L2) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_cwt_fullA<m>, grid, dim3(kPF / 8), 0, st, a, tasks, band, \
-                       tw, L1, batch, sW, sD);                 \
+    hipLaunchKernelGGL(k_cwt_fullA<m>, g8, dim3(kPF / 8), 0, st, a, tasks,         \
+                       (int)grid.x, band, tw, L1, batch, sW, sD);                  \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
@@ -620,11 +648,12 @@ int launch_fullA(int L2, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
 int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4* tasks,
                  int L2, int batch, const float2* sW, const float2* sD) {
   SSQK_GET_TW(tw, L1);
+  const dim3 g8(xcd_grid(grid.x), grid.y);
   switch (L1) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_cwt_fullB<m>, grid, dim3(kPF / 8), 0, st, a, tasks,       \
-                       tw, L2, batch, sW, sD);                 \
+    hipLaunchKernelGGL(k_cwt_fullB<m>, g8, dim3(kPF / 8), 0, st, a, tasks,         \
+                       (int)grid.x, tw, L2, batch, sW, sD);                         \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
@@ -641,11 +670,13 @@ int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup*
               const int2* tasks, const float2* coef, int coef_stride, const int2* coef_band,
               const float* x, int64_t x_stride, int pad) {
   SSQK_GET_TW(tw, F);
+  const dim3 g8(xcd_grid(grid.x), grid.y);
   switch (F) {
 #define C(m)                                                                        \
   case m:                                                                           \
-    hipLaunchKernelGGL(k_cwt_os<m>, grid, dim3(OsShape<m>::NT), 0, st, a, groups,   \
-                       tasks, coef, coef_stride, coef_band, tw, x, x_stride, pad);  \
+    hipLaunchKernelGGL(k_cwt_os<m>, g8, dim3(OsShape<m>::NT), 0, st, a, groups,     \
+                       tasks, (int)grid.x, coef, coef_stride, coef_band, tw, x,     \
+                       x_stride, pad);                                              \
     break;
     SSQK_OS_CASES(C)
 #undef C
