@@ -7,9 +7,11 @@ GMW(gamma=3, beta=60) bandpass wavelet, nv=32 log-piecewise scales
 peak maprange.  A step is one ssq_cwt call on a signal already in HBM
 producing Tx and Wx in HBM.
 
-Metric: Gscale*samples/s = na * N / step time / 1e9.  Roofline: 16 B of
-algorithmic HBM traffic per scale*sample (Wx and Tx written once as
-complex64) + 4 B per sample of x, against the 8 TB/s HBM peak.
+Metric: Gscale*samples/s = na * N / step time / 1e9.  Roofline: the
+dominant single kernel, k_squeeze, at 18 B of algorithmic HBM traffic per
+scale*sample (Wx + bin read, Tx written), timed separately with HIP events;
+the whole call's 16 B per scale*sample (x read, Wx and Tx written once) is
+reported beside it; both against the 8 TB/s HBM peak.
 CPU baseline: the reference's parallel CPU pipeline restated (scipy.fft on
 all host threads + the OpenMP port of the column-parallel squeeze loop) on
 the same workload, 1 warmup + 2 timed calls.
@@ -36,6 +38,18 @@ def chirp_tone_noise(N, seed):
     x = (np.cos(2 * np.pi * (0.02 * n + 0.09 * n**2 / N))
          + 0.5 * np.cos(2 * np.pi * 0.3 * n) + 0.1 * rng.standard_normal(N))
     return x.astype(np.float32)
+
+
+def _traffic():
+    """HBM bytes per k_squeeze launch measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy), kept in
+    profiles/squeeze_traffic.json; None if not measured."""
+    path = os.path.join(ROOT, 'profiles', 'squeeze_traffic.json')
+    try:
+        with open(path) as f:
+            return json.load(f)['traffic_bytes_per_launch']
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(x, g, reps=2):
@@ -65,6 +79,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--N', type=int, default=160000)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='end with a 256 MiB device copy (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only (this build runs the transform on a "
@@ -106,7 +122,35 @@ def main():
     alg_bytes = 4 * N + 16 * units
     achieved = alg_bytes / (ms_gpu * 1e-3) / 1e9
 
+    # the dominant single kernel, k_squeeze, timed on its own launch stream
+    # over the same inputs (Wx and bins of the last step): 10 B read (Wx
+    # complex64 + uint16 bin) + 8 B written (Tx) per scale*sample
     from ssqueezepy_amd import _lib
+    Tx, Wx = out[0], out[1]
+    bins = torch.empty(Wx.shape, dtype=torch.int16, device=dev)
+    sq_steps = max(args.steps, 5)
+    from ssqueezepy_amd._ssq_cwt import _device_const
+    from ssqueezepy_amd._cwt import run_cwt
+    run_cwt(x_dev, g['wavelet'], g['scales64'], g['dt'], 'reflect', Wx, bins=bins,
+            bp=g['bp'], gamma=10 * np.finfo(np.float32).eps, strict=True)
+    cst = _device_const(g, dev)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(sq_steps):
+        _lib.squeeze(Wx, bins, cst, na, Tx)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sq_ms = e0.elapsed_time(e1) / sq_steps
+    sq_bytes = 18 * units
+    if args.copy_calibration:
+        # plain copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+        # counters of a --pmc run (256 MiB read + 256 MiB written)
+        src = torch.empty(1 << 26, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     plan = list(_lib._plans.values())[-1]
     paths = dict(zip(('band', 'overlap_save', 'four_step'), plan.paths()))
     res = {
@@ -127,13 +171,20 @@ def main():
                                "log-piecewise ssq_freqs" % (N, na, 2 * 2**int(np.round(np.log2(N)))),
                    "batch": 1, "N": N, "na": na, "scales_per_path": paths,
                    "plan": plan.describe()},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "achieved": round(sq_bytes / sq_ms / 1e6, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "ssq_cwt step (all kernels of one call, HIP events "
-                               "on the launch stream)",
-                     "ms_gpu_per_step": round(ms_gpu, 4),
-                     "alg_bytes_per_step": alg_bytes},
+                     "frac": round(sq_bytes / sq_ms / 1e6 / HBM_PEAK_GBS, 4),
+                     "traffic": _traffic(),
+                     "kernel": "k_squeeze (ssqk_squeeze), %d launches timed with HIP "
+                               "events on the launch stream" % sq_steps,
+                     "ms_per_launch": round(sq_ms, 4),
+                     "alg_bytes_per_launch": sq_bytes,
+                     "step": {"achieved": round(achieved, 1),
+                              "frac": round(achieved / HBM_PEAK_GBS, 4),
+                              "alg_bytes": alg_bytes, "ms_gpu": round(ms_gpu, 4),
+                              "note": "whole ssq_cwt call: 4 B/sample of x read, "
+                                      "Wx and Tx written once (16 B per "
+                                      "scale*sample)"}},
     }
     if not args.no_cpu:
         cb = cpu_baseline(x, g)
