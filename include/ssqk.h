@@ -86,6 +86,20 @@ int ssqk_cwt_plan_paths(ssqk_cwt_plan plan, int64_t* counts);
  * workgroup blocks), NUL-terminated and truncated to len bytes (no
  * reference counterpart: plan introspection). */
 int ssqk_cwt_plan_describe(ssqk_cwt_plan plan, char* buf, size_t len);
+/* Adjoint of the forward transform with respect to x: given gradients gTx
+ * [batch, nf, N] and / or gWx [batch, na, N] (complex64, either may be
+ * NULL) of a real loss, writes gx [batch, N] (float32) with the squeeze
+ * rows `bins` and weights `cst` of the forward call held fixed:
+ *   ybar = gWx + cst_i gTx[bins, :],  gx = P^T Re ifft( sum_i psih_i fft(U^T ybar_i) ).
+ * No reference counterpart: the reference differentiates only cwt, through
+ * torch autograd (examples/reconstruction.py:39-42); ssq_cwt there is not
+ * differentiable.  Padding: all five modes. */
+int ssqk_cwt_adjoint(ssqk_cwt_plan plan, const void* gTx, const void* gWx,
+                     const uint16_t* bins, const float* cst, int64_t nf, int64_t batch,
+                     float* gx, void* workspace, void* stream);
+/* Device scratch bytes ssqk_cwt_adjoint needs (with_gtx: gTx will be given). */
+int ssqk_cwt_adjoint_workspace_size(ssqk_cwt_plan plan, int64_t batch, int with_gtx,
+                                    size_t* bytes);
 /* Device scratch bytes ssqk_cwt_exec needs for `batch` signals. */
 int ssqk_cwt_workspace_size(ssqk_cwt_plan plan, int64_t batch, size_t* bytes);
 

@@ -1,0 +1,84 @@
+"""Differentiable ssq_cwt and cwt: torch.autograd.Functions whose backward is
+the hand-written adjoint ``ssqk_cwt_adjoint``.
+
+The reference differentiates only ``cwt``, through torch autograd of its
+torch.fft GPU path (examples/reconstruction.py:4-7,39-42); its ``ssq_cwt``
+is not differentiable.  Here both are: ``Tx = squeeze(Wx)`` is linear in Wx
+once the squeeze rows are fixed, so the gradient of a real loss with respect
+to x is  P^T Re ifft( sum_i psih_i fft(U^T (gWx_i + c_i gTx[rows_i])) )
+with the rows of the forward call (their dependence on x through the phase
+transform is piecewise constant).  Nothing of Wx is stored for the
+backward: it needs only the uint16 rows.
+"""
+import torch
+
+
+class SsqCwtFunction(torch.autograd.Function):
+    """(x [B,] N float32 on the GPU) -> (Tx, Wx); ``run`` does the forward
+    with the library and returns (Tx, Wx, extra outputs, bins, plan, cst, nf)."""
+
+    @staticmethod
+    def forward(ctx, x, run):
+        Tx, Wx, extra, bins, plan, cst, nf = run(x.detach())
+        ctx.set_materialize_grads(False)        # unused outputs: None, not zeros
+        ctx.plan, ctx.cst, ctx.nf = plan, cst, nf
+        ctx.x_shape = x.shape
+        ctx.save_for_backward(bins)
+        for t in extra:
+            ctx.mark_non_differentiable(t)
+        return (Tx, Wx) + tuple(extra)
+
+    @staticmethod
+    def backward(ctx, gTx, gWx, *g_extra):
+        if any(g is not None and bool(torch.count_nonzero(g)) for g in g_extra):
+            raise NotImplementedError("gradients through w or dWx are not supported; "
+                                      "differentiate Tx and / or Wx")
+        (bins,) = ctx.saved_tensors
+        gTx = None if gTx is None else gTx.to(torch.complex64).contiguous()
+        gWx = None if gWx is None else gWx.to(torch.complex64).contiguous()
+        if gTx is None and gWx is None:
+            return None, None
+        ref = gTx if gTx is not None else gWx
+        gx = torch.empty(ctx.x_shape, dtype=torch.float32, device=ref.device)
+        ctx.plan.adjoint(gx, gTx=gTx, gWx=gWx,
+                         bins=bins if gTx is not None else None,
+                         cst=ctx.cst if gTx is not None else None, nf=ctx.nf)
+        return gx, None
+
+
+class CwtFunction(torch.autograd.Function):
+    """(x) -> (Wx[, dWx]) with the adjoint through Wx; dWx is an output but
+    not differentiated."""
+
+    @staticmethod
+    def forward(ctx, x, run):
+        Wx, extra, plan = run(x.detach())
+        ctx.set_materialize_grads(False)
+        ctx.plan = plan
+        ctx.x_shape = x.shape
+        for t in extra:
+            ctx.mark_non_differentiable(t)
+        return (Wx,) + tuple(extra)
+
+    @staticmethod
+    def backward(ctx, gWx, *g_extra):
+        if any(g is not None and bool(torch.count_nonzero(g)) for g in g_extra):
+            raise NotImplementedError("gradients through dWx are not supported")
+        if gWx is None:
+            return None, None
+        gWx = gWx.to(torch.complex64).contiguous()
+        gx = torch.empty(ctx.x_shape, dtype=torch.float32, device=gWx.device)
+        ctx.plan.adjoint(gx, gWx=gWx)
+        return gx, None
+
+
+def wants_grad(x):
+    return isinstance(x, torch.Tensor) and x.requires_grad and torch.is_grad_enabled()
+
+
+def differentiable_input(x, dtype):
+    """x moved to the GPU as float32 inside the autograd graph."""
+    if dtype != 'float32':
+        raise NotImplementedError("only float32 transforms run on the GPU path")
+    dev = x.device if x.is_cuda else torch.device('cuda', torch.cuda.current_device())
+    return x.to(device=dev, dtype=torch.float32).contiguous()

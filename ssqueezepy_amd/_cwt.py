@@ -11,6 +11,7 @@ from ._common import WARN, _process_fs_and_t, p2up
 from ._scales import process_scales
 from ._tensors import device_constant, signal_to_device, to_output
 from .wavelets import Wavelet, KIND_GMW
+from ._autograd import CwtFunction, differentiable_input, wants_grad
 
 __all__ = ['cwt']
 
@@ -80,20 +81,33 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
                         "(got %s)" % type(x))
     elif x.ndim not in (1, 2):
         raise ValueError("`x` must be 1D or 2D (got x.ndim == %s)" % x.ndim)
-    x = _nan_checks(x, nan_checks)
+    grad = wants_grad(x)
+    if not grad:
+        x = _nan_checks(x, nan_checks)
     if not isinstance(scales, str):
         nv = None
     N = x.shape[-1]
     dt, *_ = _process_fs_and_t(fs, t, N=N)
     wavelet = Wavelet._init_if_not_isinstance(wavelet)
-    x_dev = signal_to_device(x, wavelet.dtype)
     scales = process_scales(scales, N, wavelet, nv=nv)
     scales64 = np.asarray(scales, dtype=np.float64).ravel()
     na = len(scales64)
-    shape = (na, N) if x.ndim == 1 else (x.shape[0], na, N)
-    Wx = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
-    dWx = torch.empty_like(Wx) if derivative else None
-    run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=dWx)
+
+    def run(x_dev):
+        shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
+        Wx = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
+        dWx = torch.empty_like(Wx) if derivative else None
+        plan = run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=dWx)
+        return Wx, ((dWx,) if derivative else ()), plan
+
+    if grad:
+        x_dev = differentiable_input(x, wavelet.dtype)
+        outs = CwtFunction.apply(x_dev, run)
+        Wx, dWx = outs[0], (outs[1] if derivative else None)
+    else:
+        x_dev = signal_to_device(x, wavelet.dtype)
+        Wx, extra, _ = run(x_dev)
+        dWx = extra[0] if derivative else None
     scales_out = device_constant(scales64.astype(wavelet.dtype), torch.float32,
                                  x_dev.device, copy=True)
     Wx, scales_out, dWx = [to_output(g, astensor) for g in (Wx, scales_out, dWx)]

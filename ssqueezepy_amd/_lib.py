@@ -46,6 +46,10 @@ _SIGS = {
     'ssqk_cwt_exec': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                               c_void_p, c_void_p, ctypes.POINTER(BinParams),
                               c_double, c_int, c_void_p, c_void_p]),
+    'ssqk_cwt_adjoint': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    'ssqk_cwt_adjoint_workspace_size': (c_int, [c_void_p, c_int64, c_int,
+                                                ctypes.POINTER(c_size_t)]),
     'ssqk_squeeze': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                              c_int64, c_int64, c_void_p, c_void_p]),
     'ssqk_bins_from_dwx': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64,
@@ -170,6 +174,20 @@ class CwtPlan:
             ctypes.byref(bp) if bp is not None else None, float(gamma),
             int(bool(strict)), _ptr(workspace), _stream(x.device)))
         return workspace
+
+    def adjoint(self, gx, gTx=None, gWx=None, bins=None, cst=None, nf=0):
+        """gx = d loss / d x through Wx (gWx) and Tx (gTx, with the forward's
+        squeeze rows ``bins`` and weights ``cst`` held fixed)."""
+        import torch
+        batch = 1 if gx.ndim == 1 else gx.shape[0]
+        n = c_size_t()
+        check(load().ssqk_cwt_adjoint_workspace_size(self._h, batch, int(gTx is not None),
+                                                     ctypes.byref(n)))
+        ws = torch.empty(n.value, dtype=torch.uint8, device=gx.device)
+        check(load().ssqk_cwt_adjoint(
+            self._h, _ptr(gTx), _ptr(gWx), _ptr(bins), _ptr(cst), int(nf), batch,
+            _ptr(gx), _ptr(ws), _stream(gx.device)))
+        return gx
 
     def __del__(self):
         h = getattr(self, '_h', None)

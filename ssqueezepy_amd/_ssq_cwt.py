@@ -16,6 +16,7 @@ from .ssqueezing import (_apply_squeezing, _check_ssqueezing_args,
                          bin_params_from_freqs, compute_associated_frequencies,
                          cwt_const)
 from .wavelets import Wavelet
+from ._autograd import SsqCwtFunction, differentiable_input, wants_grad
 
 __all__ = ['ssq_cwt', 'phase_cwt', 'ssq_cwt_grid']
 
@@ -141,29 +142,52 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
         raise NotImplementedError("higher-order GMW ssq_cwt is not on the GPU path")
     difforder = _check_ssqueezing_args(squeezing, maprange, wavelet, difftype,
                                        difforder, get_w, transform='cwt')
-    x = _nan_checks(x, nan_checks)
+    grad = wants_grad(x)
+    if not grad:
+        x = _nan_checks(x, nan_checks)
     N = x.shape[-1]
     g = ssq_cwt_grid(N, wavelet, scales, nv, fs, t, ssq_freqs, maprange,
                      padtype, flipud)
     wavelet = g['wavelet']
-    x_dev = signal_to_device(x, wavelet.dtype)
-    dev = x_dev.device
     na = g['na']
-    shape = (na, N) if x.ndim == 1 else (x.shape[0], na, N)
     if gamma is None:
         gamma = 10 * EPS32
 
-    Wx = torch.empty(shape, dtype=torch.complex64, device=dev)
-    dWx = torch.empty_like(Wx) if get_dWx else None
-    w = torch.empty(shape, dtype=torch.float32, device=dev) if get_w else None
-    bins = torch.empty(shape, dtype=torch.int16, device=dev)
-    # get_w: phase_cwt then indexed_sum_onfly (w = inf where |Wx| < gamma);
-    # otherwise the fused ssqueeze_fast test |Wx| > gamma
-    run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx, w=w,
-            bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w)
-    Wsq = _apply_squeezing(Wx, squeezing).contiguous()
-    Tx = torch.empty_like(Wx)
-    _lib.squeeze(Wsq, bins, _device_const(g, dev), na, Tx)
+    def run(x_dev):
+        dev = x_dev.device
+        shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
+        Wx = torch.empty(shape, dtype=torch.complex64, device=dev)
+        dWx = torch.empty_like(Wx) if get_dWx else None
+        w = torch.empty(shape, dtype=torch.float32, device=dev) if get_w else None
+        bins = torch.empty(shape, dtype=torch.int16, device=dev)
+        # get_w: phase_cwt then indexed_sum_onfly (w = inf where |Wx| < gamma);
+        # otherwise the fused ssqueeze_fast test |Wx| > gamma
+        plan = run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx,
+                       w=w, bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w)
+        Wsq = _apply_squeezing(Wx, squeezing).contiguous()
+        Tx = torch.empty_like(Wx)
+        cst = _device_const(g, dev)
+        _lib.squeeze(Wsq, bins, cst, na, Tx)
+        extra = tuple(v for v in (w, dWx) if v is not None)
+        return Tx, Wx, extra, bins, plan, cst, na
+
+    if grad:
+        if squeezing != 'sum':
+            raise NotImplementedError("gradients need squeezing='sum' (Tx linear in Wx)")
+        x_dev = differentiable_input(x, wavelet.dtype)
+        outs = SsqCwtFunction.apply(x_dev, run)
+        Tx, Wx = outs[0], outs[1]
+        rest = list(outs[2:])
+        w = rest.pop(0) if get_w else None
+        dWx = rest.pop(0) if get_dWx else None
+        dev = x_dev.device
+    else:
+        x_dev = signal_to_device(x, wavelet.dtype)
+        dev = x_dev.device
+        Tx, Wx, extra, *_ = run(x_dev)
+        extra = list(extra)
+        w = extra.pop(0) if get_w else None
+        dWx = extra.pop(0) if get_dWx else None
 
     ssq_freqs = g['ssq_freqs'][::-1]          # ssqueezing.py:216-222
     scales_out = device_constant(g['scales'], torch.float32, dev, copy=True) \

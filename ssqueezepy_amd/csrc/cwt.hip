@@ -883,6 +883,22 @@ struct ssqk_cwt_plan_s {
   int path_count[3] = {0, 0, 0};   // band, overlap-save, four-step
   // side streams: the scale classes are spread over the caller's stream and
   // these (fork after the forward FFT / at entry, join before returning)
+  // adjoint (gradient of Wx / Tx with respect to x): band classes of the
+  // transposed band transform, the scales whose band exceeds kP (four-step),
+  // and per scale the window {ka, len} of its psih table and spectrum
+  struct AdjCls {
+    int M = 0, ph = 1;        // band size; phase blocks per workgroup
+    int ntasks = 0, nred = 0;
+    int4* d_tasks = nullptr;  // {scale, first phase, ka, log2 D}
+    int4* d_red = nullptr;    // {scale, first task, tasks, -}
+  };
+  std::vector<AdjCls> adj;
+  std::vector<int> adj_wide;
+  int adj_slots = 0;
+  std::vector<Chunk> adj_chunks;
+  int2* d_adj_seg = nullptr;
+  int64_t tab_total = 0;
+  int64_t adj_partial = 0;    // complex elements of the largest class's partial sums
   static constexpr int kSide = 3;
   int nside = kSide;          // side streams in use (SSQK_SIDE_STREAMS, 0..3)
   hipStream_t side[kSide] = {};
@@ -1012,6 +1028,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
   std::vector<int> cls_scales(14, 0);
   std::vector<int> os_F(na, 0), os_H(na, 0);
   std::vector<int64_t> tab_len(na, 0);
+  std::vector<int> adj_M(na, 0);
   for (int64_t i = 0; i < na; ++i) {
     const double s = scales[i];
     if (!(s > 0)) {
@@ -1042,6 +1059,8 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
         if (c < cost) { cost = c; bestF = F; bestH = H; }
       }
     }
+    tab_len[i] = M <= kP ? M : K;            // psih window (also the adjoint's)
+    adj_M[i] = M <= kP ? (int)M : 0;
     if (bestF) {
       os_F[i] = bestF;
       os_H[i] = bestH;
@@ -1054,11 +1073,9 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       for (int64_t a0 = 0; a0 < D; a0 += rows)
         cls_tasks[lM].push_back(make_int4((int)i, (int)a0, (int)ka, ilog2(D)));
       cls_scales[lM]++;
-      tab_len[i] = M;
       p->path_count[0]++;
     } else {
       p->full.push_back((int)i);
-      tab_len[i] = K;
       p->path_count[2]++;
     }
   }
@@ -1128,6 +1145,65 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       }
       if (e == hipSuccess) e = hipDeviceSynchronize();
       (void)hipFree(d_segs);
+    }
+  }
+
+  // adjoint task lists
+  if (e == hipSuccess) {
+    std::vector<int2> seg(na);
+    int64_t off = 0;
+    for (int64_t i = 0; i < na; ++i) {
+      seg[i] = make_int2(band[i].x, (int)tab_len[i]);
+      off += tab_len[i];
+    }
+    p->tab_total = off;
+    e = upload(&p->d_adj_seg, seg);
+    for (int lM = 1; lM < 14 && e == hipSuccess; ++lM) {
+      const int M = 1 << lM;
+      std::vector<int4> tasks, red;
+      const int64_t D = L / M;
+      const int rows = band_rows(M);
+      const int ph = (int)std::max<int64_t>(1, std::min<int64_t>(8, D / rows));
+      for (int64_t i = 0; i < na; ++i) {
+        if (adj_M[i] != M) continue;
+        const int t0 = (int)tasks.size();
+        for (int64_t a0 = 0; a0 < D; a0 += (int64_t)rows * ph)
+          tasks.push_back(make_int4((int)i, (int)a0, band[i].x, ilog2(D)));
+        red.push_back(make_int4((int)i, t0, (int)tasks.size() - t0, 0));
+      }
+      if (tasks.empty()) continue;
+      ssqk_cwt_plan_s::AdjCls c;
+      c.M = M;
+      c.ph = ph;
+      c.ntasks = (int)tasks.size();
+      c.nred = (int)red.size();
+      e = upload(&c.d_tasks, tasks);
+      if (e == hipSuccess) e = upload(&c.d_red, red);
+      p->adj.push_back(c);
+      p->adj_partial = std::max<int64_t>(p->adj_partial, (int64_t)c.ntasks * M);
+    }
+    for (int64_t i = 0; i < na; ++i)
+      if (!adj_M[i]) p->adj_wide.push_back((int)i);
+    if (!p->adj_wide.empty() && e == hipSuccess) {
+      if (!p->L1) split_four_step(L, &p->L1, &p->L2);
+      const int64_t per_slot = L * (int64_t)sizeof(float2);
+      p->adj_slots = (int)std::max<int64_t>(
+          1, std::min<int64_t>((int64_t)p->adj_wide.size(), (64ll << 20) / per_slot));
+      const int GA = kPF / p->L2, GB = kPF / p->L1;
+      for (size_t c0 = 0; c0 < p->adj_wide.size() && e == hipSuccess; c0 += p->adj_slots) {
+        std::vector<int4> A, B;
+        for (int sl = 0; sl < p->adj_slots && c0 + sl < p->adj_wide.size(); ++sl) {
+          const int i = p->adj_wide[c0 + sl];
+          for (int k1 = 0; k1 < p->L1; k1 += GA) A.push_back(make_int4(i, k1, sl, 0));
+          for (int n2 = 0; n2 < p->L2; n2 += GB) B.push_back(make_int4(i, n2, sl, 0));
+        }
+        ssqk_cwt_plan_s::Chunk ch;
+        ch.nA = (int)A.size();
+        ch.nB = (int)B.size();
+        e = upload(&ch.d_A, A);
+        if (e == hipSuccess) e = upload(&ch.d_B, B);
+        p->adj_chunks.push_back(ch);
+      }
     }
   }
 
@@ -1294,6 +1370,15 @@ extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
   (void)hipFree(p->d_psih);
   (void)hipFree(p->d_psih_off);
   (void)hipFree(p->d_coef_band);
+  for (auto& c : p->adj) {
+    (void)hipFree(c.d_tasks);
+    (void)hipFree(c.d_red);
+  }
+  for (auto& c : p->adj_chunks) {
+    (void)hipFree(c.d_A);
+    (void)hipFree(c.d_B);
+  }
+  (void)hipFree(p->d_adj_seg);
   for (int k = 0; k < ssqk_cwt_plan_s::kSide; ++k) {
     if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
     if (p->ev_done[k]) (void)hipEventDestroy(p->ev_done[k]);
@@ -1499,4 +1584,595 @@ extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, voi
   }
   return exec_impl(p, x, batch, Wx, dWx, w, bins, bp, gamma, gamma_strict, workspace,
                    (hipStream_t)stream);
+}
+
+// =========================================================== adjoint
+// Gradient of a real loss through Wx = cwt(x) and Tx = squeeze(Wx) with the
+// squeeze rows held fixed.  With Wx_i = U ifft_L(psih_i . fft_L(P x)) (P the
+// padding, U the unpadding), the adjoint is
+//   xbar = P^T Re( ifft_L( sum_i psih_i . fft_L(U^T ybar_i) ) ),
+//   ybar_i[j] = gWx[i, j] + c_i gTx[row(i, j), j]        (squeeze transposed)
+// psih_i is real and vanishes outside the scale's band, so fft_L(U^T ybar_i)
+// is needed on the band only: for bands up to kP bins the transposed band
+// transform (decimation in time over the D = L/M phases), wider bands by a
+// four-step FFT.  Each band spectrum Y_i lands in a table laid out like the
+// psih tables; the spectra are summed over scales in ascending order per bin
+// (deterministic), inverse-FFT'd once and folded back through the padding.
+namespace {
+
+struct AdjArgs {
+  const float2* yb;          // ybar [batch][na][N]
+  int64_t yb_stride;         // na * N
+  int64_t N, n1;
+  int L;
+  int na;
+  const float2* ptab;
+  int pS;
+  const float* psih;         // psih tables, scale i at psih_off[i]
+  const int64_t* psih_off;
+  float2* ytab;              // band spectra, same layout, [batch][tab_total]
+  int64_t ytab_stride;
+};
+
+// ybar = gWx + c_i gTx[row, j]: a workgroup holds the gTx columns
+// [j0, j0 + W) of all nf rows in LDS and gathers from them.
+template <int W>
+__global__ __launch_bounds__(256) void k_adj_ybar(const float2* __restrict__ gTx,
+                                                  const float2* __restrict__ gWx,
+                                                  const uint16_t* __restrict__ bins,
+                                                  const float* __restrict__ cst, int na, int nf,
+                                                  int64_t N, float2* __restrict__ yb) {
+  extern __shared__ float2 s_tx[];           // [nf][W]
+  const int b = blockIdx.y;
+  const int64_t j0 = (int64_t)blockIdx.x * W;
+  const float2* tx = gTx + (int64_t)b * nf * N;
+  for (int e = threadIdx.x; e < nf * W; e += 256) {
+    const int k = e / W, c = e - k * W;
+    const int64_t j = j0 + c;
+    s_tx[e] = tx[(int64_t)k * N + (j < N ? j : N - 1)];
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)b * na * N;
+  for (int e = threadIdx.x; e < na * W; e += 256) {
+    const int i = e / W, c = e - i * W;
+    const int64_t j = j0 + c;
+    if (j >= N) continue;
+    const int64_t o = base + (int64_t)i * N + j;
+    float2 v = gWx ? gWx[o] : make_float2(0.f, 0.f);
+    const int k = bins[o];
+    if (k != kNoBin && k < nf) {
+      const float2 t = s_tx[k * W + c];
+      const float ci = cst[i];
+      v.x += ci * t.x;
+      v.y += ci * t.y;
+    }
+    yb[o] = v;
+  }
+}
+
+// Transposed band transform of scale i over the phases [a0, a0 + ph*ROWS):
+//   Y[ka + m] = sum_a w_L^{-(ka+m) a} FFT_M( u[a + D q] w_M^{-ka q} )[m],
+// u = ybar_i placed at the padded positions [n1, n1 + N).  One partial
+// M-vector per task, summed over its phases in a fixed order.
+template <int M>
+__global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
+                                                             const int4* __restrict__ tasks,
+                                                             int ntasks, int ph,
+                                                             const float2* __restrict__ tw,
+                                                             float2* __restrict__ partial) {
+  constexpr int ROWS = band_rows(M);
+  constexpr int NT = band_threads(M);
+  constexpr int IT = ROWS * M / NT;          // 8
+  __shared__ float2 s[Lds<M>::elems(ROWS)];
+  const int task = xcd_task(blockIdx.x, ntasks);
+  if (task < 0) return;
+  const int4 t = tasks[task];
+  const int b = blockIdx.y;
+  const int i = t.x, a0 = t.y, ka = t.z, lD = t.w;
+  const int64_t D = (int64_t)1 << lD;
+  const float2* yb = a.yb + (int64_t)b * a.yb_stride + (int64_t)i * a.N;
+  const int L = a.L;
+  float2 acc[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) acc[it] = make_float2(0.f, 0.f);
+  for (int p = 0; p < ph; ++p) {
+    const int64_t ab = a0 + (int64_t)p * ROWS;
+    if (ab >= D) break;                      // uniform
+    float2 v[IT];
+    float2 w[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      const int r = e / M, q = e - r * M;
+      const int64_t j = ab + r + D * q - a.n1;
+      v[it] = yb[(j >= 0 && j < a.N) ? j : 0];
+      w[it] = cconj(phase_tab((uint64_t)ka * (uint64_t)q * (uint64_t)D, a.ptab, L, a.pS));
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      const int r = e / M, q = e - r * M;
+      const int64_t j = ab + r + D * q - a.n1;
+      const bool in = j >= 0 && j < a.N && ab + r < D;
+      s[Lds<M>::idx(r, q)] = in ? cmul(v[it], w[it]) : make_float2(0.f, 0.f);
+    }
+    float2* const bufs[1] = {s};
+    lds_fft<M, ROWS, -1, NT, 1>(bufs, tw);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      const int r = e / M, m = e - r * M;
+      const float2 ph_ = cconj(phase_tab((uint64_t)(ka + m) * (uint64_t)(ab + r), a.ptab, L,
+                                         a.pS));
+      const float2 y = cmul(s[Lds<M>::idx(r, m)], ph_);
+      acc[it].x += y.x;
+      acc[it].y += y.y;
+    }
+    __syncthreads();
+  }
+  float2* out = partial + ((int64_t)b * ntasks + task) * M;
+  if constexpr (ROWS == 1) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) out[threadIdx.x + it * NT] = acc[it];
+  } else {
+    // sum the rows of each bin m in row order through LDS
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      const int r = e / M, m = e - r * M;
+      s[Lds<M>::idx(r, m)] = acc[it];
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < M; m += NT) {
+      float2 v = make_float2(0.f, 0.f);
+      for (int r = 0; r < ROWS; ++r) {
+        const float2 q = s[Lds<M>::idx(r, m)];
+        v.x += q.x;
+        v.y += q.y;
+      }
+      out[m] = v;
+    }
+  }
+}
+
+// Y_i[m] = sum of scale i's partials (its tasks in order).
+__global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a, const int4* __restrict__ red,
+                                                    int M, int ntasks,
+                                                    const float2* __restrict__ partial) {
+  const int4 r = red[blockIdx.x];
+  const int b = blockIdx.y;
+  const float2* pb = partial + ((int64_t)b * ntasks + r.y) * M;
+  float2* y = a.ytab + (int64_t)b * a.ytab_stride + a.psih_off[r.x];
+  for (int m = threadIdx.x; m < M; m += 256) {
+    float2 v = make_float2(0.f, 0.f);
+    for (int t = 0; t < r.z; ++t) {
+      const float2 q = pb[(int64_t)t * M + m];
+      v.x += q.x;
+      v.y += q.y;
+    }
+    y[m] = v;
+  }
+}
+
+// Four-step forward FFT of u = ybar_i at the padded positions for the
+// scales with bands wider than kP; band bins go to the spectrum table.
+constexpr int kAdjNT4 = kPF / 16;             // one radix-16 butterfly per thread
+
+template <int L2>
+__global__ __launch_bounds__(kAdjNT4) void k_adj_fullA(AdjArgs a, const int4* __restrict__ tasks,
+                                                       int ntasks, const float2* __restrict__ tw,
+                                                       int L1, int batch,
+                                                       float2* __restrict__ scr) {
+  constexpr int G = kPF / L2;
+  __shared__ float2 s[Lds<L2>::elems(G)];
+  const int task = xcd_task(blockIdx.x, ntasks);
+  if (task < 0) return;
+  const int4 t = tasks[task];
+  const int b = blockIdx.y;
+  const int i = t.x, k1_0 = t.y, slot = t.z;
+  const float2* yb = a.yb + (int64_t)b * a.yb_stride + (int64_t)i * a.N;
+  for (int e = threadIdx.x; e < G * L2; e += kAdjNT4) {
+    const int g = e % G, k2 = e / G;
+    const int64_t j = k1_0 + g + (int64_t)L1 * k2 - a.n1;
+    const float2 v = yb[(j >= 0 && j < a.N) ? j : 0];
+    s[Lds<L2>::idx(g, k2)] = (j >= 0 && j < a.N) ? v : make_float2(0.f, 0.f);
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L2, G, -1, kAdjNT4, 1>(bufs, tw);
+  const int64_t base = ((int64_t)slot * batch + b) * (int64_t)L2;
+  for (int e = threadIdx.x; e < G * L2; e += kAdjNT4) {
+    const int g = e % G, n2 = e / G;
+    const float2 ph = cconj(phase_tab((uint64_t)(k1_0 + g) * (uint64_t)n2, a.ptab, a.L, a.pS));
+    scr[(base + n2) * L1 + k1_0 + g] = cmul(s[Lds<L2>::idx(g, n2)], ph);
+  }
+}
+
+template <int L1>
+__global__ __launch_bounds__(kAdjNT4) void k_adj_fullB(AdjArgs a, const int4* __restrict__ tasks,
+                                                       int ntasks, const int2* __restrict__ seg,
+                                                       const float2* __restrict__ tw, int L2,
+                                                       int batch,
+                                                       const float2* __restrict__ scr) {
+  constexpr int G = kPF / L1;
+  __shared__ float2 s[Lds<L1>::elems(G)];
+  const int task = xcd_task(blockIdx.x, ntasks);
+  if (task < 0) return;
+  const int4 t = tasks[task];
+  const int b = blockIdx.y;
+  const int i = t.x, n2_0 = t.y, slot = t.z;
+  const int64_t base = ((int64_t)slot * batch + b) * (int64_t)L2;
+  for (int e = threadIdx.x; e < G * L1; e += kAdjNT4) {
+    const int k1 = e % L1, g = e / L1;
+    s[Lds<L1>::idx(g, k1)] = scr[(base + n2_0 + g) * L1 + k1];
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L1, G, -1, kAdjNT4, 1>(bufs, tw);
+  const int2 sg = seg[i];
+  float2* y = a.ytab + (int64_t)b * a.ytab_stride + a.psih_off[i];
+  for (int e = threadIdx.x; e < G * L1; e += kAdjNT4) {
+    const int g = e % G, n1 = e / G;
+    const int64_t k = (int64_t)L2 * n1 + n2_0 + g;
+    const int64_t m = k - sg.x;
+    if (m >= 0 && m < sg.y) y[m] = s[Lds<L1>::idx(g, n1)];
+  }
+}
+
+// S[k] = sum_i psih_i[k] Y_i[k] over the scales whose window holds k, in
+// ascending scale order; zero above the Nyquist bin.
+__global__ __launch_bounds__(256) void k_adj_accum(AdjArgs a, const int2* __restrict__ seg,
+                                                   float2* __restrict__ S) {
+  const int b = blockIdx.y;
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= a.L) return;
+  float2 acc = make_float2(0.f, 0.f);
+  if (k <= a.L / 2) {
+    const float2* yt = a.ytab + (int64_t)b * a.ytab_stride;
+    for (int i = 0; i < a.na; ++i) {
+      const int2 sg = seg[i];
+      const int64_t m = k - sg.x;
+      if (m >= 0 && m < sg.y) {
+        const int64_t o = a.psih_off[i] + m;
+        const float ps = a.psih[o];
+        const float2 y = yt[o];
+        acc.x += ps * y.x;
+        acc.y += ps * y.y;
+      }
+    }
+  }
+  S[(int64_t)b * a.L + k] = acc;
+}
+
+// xp = Re(ifft_L(S)) (one workgroup for L <= kSmallFwd, else four-step)
+template <int L>
+__global__ __launch_bounds__(kNT) void k_adj_ifft_small(const float2* __restrict__ S,
+                                                        const float2* __restrict__ tw,
+                                                        float* __restrict__ xp) {
+  __shared__ float2 s[Lds<L>::elems(1)];
+  const int b = blockIdx.x;
+  for (int k = threadIdx.x; k < L; k += kNT) s[Lds<L>::idx(0, k)] = S[(int64_t)b * L + k];
+  float2* const bufs[1] = {s};
+  lds_fft<L, 1, +1, kNT, 1>(bufs, tw);
+  const float inv = 1.0f / (float)L;
+  for (int n = threadIdx.x; n < L; n += kNT) xp[(int64_t)b * L + n] = s[Lds<L>::idx(0, n)].x * inv;
+}
+
+template <int L2>
+__global__ __launch_bounds__(kNT) void k_adj_ifftA(const float2* __restrict__ S,
+                                                   const float2* __restrict__ tw, int L1,
+                                                   const float2* __restrict__ ptab, int pS,
+                                                   float2* __restrict__ scr) {
+  constexpr int G = kP / L2;
+  __shared__ float2 s[Lds<L2>::elems(G)];
+  const int b = blockIdx.y;
+  const int k1_0 = blockIdx.x * G;
+  const int L = L1 * L2;
+  for (int e = threadIdx.x; e < G * L2; e += kNT) {
+    const int g = e % G, k2 = e / G;
+    s[Lds<L2>::idx(g, k2)] = S[(int64_t)b * L + k1_0 + g + (int64_t)L1 * k2];
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L2, G, +1, kNT, 1>(bufs, tw);
+  for (int e = threadIdx.x; e < G * L2; e += kNT) {
+    const int g = e % G, n2 = e / G;
+    const int k1 = k1_0 + g;
+    const float2 ph = phase_tab((uint64_t)k1 * (uint64_t)n2, ptab, L, pS);
+    scr[((int64_t)b * L2 + n2) * L1 + k1] = cmul(s[Lds<L2>::idx(g, n2)], ph);
+  }
+}
+
+template <int L1>
+__global__ __launch_bounds__(kNT) void k_adj_ifftB(const float2* __restrict__ scr,
+                                                   const float2* __restrict__ tw, int L2,
+                                                   float* __restrict__ xp) {
+  constexpr int G = kP / L1;
+  __shared__ float2 s[Lds<L1>::elems(G)];
+  const int b = blockIdx.y;
+  const int n2_0 = blockIdx.x * G;
+  const int L = L1 * L2;
+  for (int e = threadIdx.x; e < G * L1; e += kNT) {
+    const int k1 = e % L1, g = e / L1;
+    s[Lds<L1>::idx(g, k1)] = scr[((int64_t)b * L2 + n2_0 + g) * L1 + k1];
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L1, G, +1, kNT, 1>(bufs, tw);
+  const float inv = 1.0f / (float)L;
+  for (int e = threadIdx.x; e < G * L1; e += kNT) {
+    const int g = e % G, n1 = e / G;
+    xp[(int64_t)b * L + (int64_t)L2 * n1 + n2_0 + g] = s[Lds<L1>::idx(g, n1)].x * inv;
+  }
+}
+
+// gx[t] = sum of xp[n] over the padded positions n whose source is t
+// (the transpose of pad_src).
+__global__ __launch_bounds__(256) void k_adj_fold(const float* __restrict__ xp, int64_t L,
+                                                  int64_t N, int64_t n1, int pad,
+                                                  float* __restrict__ gx) {
+  const int b = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= N) return;
+  const float* x = xp + (int64_t)b * L;
+  float acc = 0.f;
+  auto add_progression = [&](int64_t base, int64_t P) {
+    // every n = base + k P in [0, L)
+    int64_t k0 = base >= 0 ? -(base / P) : (-base + P - 1) / P;
+    for (int64_t n = base + k0 * P; n < L; n += P)
+      if (n >= 0) acc += x[n];
+  };
+  switch (pad) {
+    case SSQK_PAD_REFLECT:
+      if (N == 1) {
+        for (int64_t n = 0; n < L; ++n) acc += x[n];
+      } else {
+        const int64_t P = 2 * (N - 1);
+        add_progression(n1 + t, P);
+        if (t != 0 && t != N - 1) add_progression(n1 - t, P);
+      }
+      break;
+    case SSQK_PAD_SYMMETRIC:
+      add_progression(n1 + t, 2 * N);
+      add_progression(n1 - 1 - t, 2 * N);
+      break;
+    case SSQK_PAD_WRAP:
+      add_progression(n1 + t, N);
+      break;
+    case SSQK_PAD_REPLICATE:
+      acc = x[n1 + t];
+      if (t == 0)
+        for (int64_t n = 0; n < n1; ++n) acc += x[n];
+      if (t == N - 1)
+        for (int64_t n = n1 + N; n < L; ++n) acc += x[n];
+      break;
+    default:                                 // zero padding
+      acc = x[n1 + t];
+  }
+  gx[(int64_t)b * N + t] = acc;
+}
+
+int launch_adj_band(int M, dim3 grid, hipStream_t st, const AdjArgs& a, const int4* tasks,
+                    int ph, float2* partial) {
+  SSQK_GET_TW(tw, M);
+  const dim3 g8(xcd_grid(grid.x), grid.y);
+  switch (M) {
+#define C(m)                                                                        \
+  case m:                                                                           \
+    hipLaunchKernelGGL(k_adj_band<m>, g8, dim3(band_threads(m)), 0, st, a, tasks,   \
+                       (int)grid.x, ph, tw, partial);                               \
+    break;
+    SSQK_POW2_CASES(C)
+#undef C
+    default:
+      ssqk_host::set_error("adjoint band size %d unsupported", M);
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  return SSQK_OK;
+}
+
+void adj_ws_layout(const ssqk_cwt_plan_s* p, int64_t batch, bool need_yb, size_t off[6],
+                   size_t* total) {
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  size_t o = 0;
+  off[0] = o;                                        // ybar
+  if (need_yb) o = al(o + sizeof(float2) * (size_t)(batch * p->na * p->N));
+  off[1] = o;                                        // band spectra
+  o = al(o + sizeof(float2) * (size_t)(batch * std::max<int64_t>(p->tab_total, 1)));
+  off[2] = o;                                        // partial sums
+  o = al(o + sizeof(float2) * (size_t)(batch * std::max<int64_t>(p->adj_partial, 1)));
+  off[3] = o;                                        // four-step scratch
+  if (!p->adj_wide.empty()) o = al(o + sizeof(float2) * (size_t)(batch * p->adj_slots * p->L));
+  off[4] = o;                                        // S, then ifft scratch
+  o = al(o + sizeof(float2) * (size_t)(2 * batch * p->L));
+  off[5] = o;                                        // xp
+  o = al(o + sizeof(float) * (size_t)(batch * p->L));
+  *total = o;
+}
+
+int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
+                 const uint16_t* bins, const float* cst, int64_t nf, int64_t batch,
+                 float* gx, void* workspace, hipStream_t st) {
+  const bool need_yb = gTx != nullptr;
+  size_t off[6], total;
+  adj_ws_layout(p, batch, need_yb, off, &total);
+  char* ws = (char*)workspace;
+  const int64_t N = p->N, L = p->L;
+  AdjArgs a;
+  a.yb = need_yb ? (const float2*)(ws + off[0]) : gWx;
+  a.yb_stride = p->na * N;
+  a.N = N;
+  a.n1 = p->n1;
+  a.L = (int)L;
+  a.na = (int)p->na;
+  a.ptab = ssqk_host::phase_table((int)L);
+  a.pS = phase_tab_split(L);
+  a.psih = p->d_psih;
+  a.psih_off = p->d_psih_off;
+  a.ytab = (float2*)(ws + off[1]);
+  a.ytab_stride = std::max<int64_t>(p->tab_total, 1);
+  if (!a.ptab) {
+    ssqk_host::set_error("could not create the phase table");
+    return SSQK_ERR_HIP;
+  }
+  if (need_yb) {
+    constexpr size_t kLds = 160 * 1024;
+    int W = 32;
+    while (W > 8 && (size_t)nf * W * sizeof(float2) > kLds) W /= 2;
+    if ((size_t)nf * W * sizeof(float2) > kLds) {
+      ssqk_host::set_error("nf=%lld rows too many for the adjoint gather", (long long)nf);
+      return SSQK_ERR_UNSUPPORTED;
+    }
+    const size_t lds = (size_t)nf * W * sizeof(float2);
+    dim3 grid((unsigned)((N + W - 1) / W), (unsigned)batch);
+    float2* yb = (float2*)(ws + off[0]);
+    static bool attr[3] = {false, false, false};
+    auto go = [&](auto kern, int wi) -> int {
+      if (!attr[wi]) {
+        SSQK_CHECK(hipFuncSetAttribute((const void*)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds));
+        attr[wi] = true;
+      }
+      hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, gTx, gWx, bins, cst, (int)p->na,
+                         (int)nf, N, yb);
+      SSQK_CHECK(hipGetLastError());
+      return SSQK_OK;
+    };
+    const int rc = W == 32 ? go(k_adj_ybar<32>, 0) : W == 16 ? go(k_adj_ybar<16>, 1)
+                                                             : go(k_adj_ybar<8>, 2);
+    if (rc) return rc;
+  }
+  float2* partial = (float2*)(ws + off[2]);
+  for (const auto& c : p->adj) {
+    int rc = launch_adj_band(c.M, dim3(c.ntasks, (unsigned)batch), st, a, c.d_tasks, c.ph,
+                             partial);
+    if (rc) return rc;
+    SSQK_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_adj_reduce, dim3(c.nred, (unsigned)batch), dim3(256), 0, st, a,
+                       c.d_red, c.M, c.ntasks, partial);
+    SSQK_CHECK(hipGetLastError());
+  }
+  if (!p->adj_chunks.empty()) {
+    float2* scr = (float2*)(ws + off[3]);
+    for (const auto& ch : p->adj_chunks) {
+      {
+        SSQK_GET_TW(tw, p->L2);
+        const dim3 g8(xcd_grid(ch.nA), (unsigned)batch);
+        switch (p->L2) {
+#define C(m)                                                                          \
+  case m:                                                                             \
+    hipLaunchKernelGGL(k_adj_fullA<m>, g8, dim3(kAdjNT4), 0, st, a, ch.d_A, ch.nA, tw, \
+                       p->L1, (int)batch, scr);                                       \
+    break;
+          C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+          default:
+            ssqk_host::set_error("four-step size %d unsupported", p->L2);
+            return SSQK_ERR_UNSUPPORTED;
+        }
+        SSQK_CHECK(hipGetLastError());
+      }
+      {
+        SSQK_GET_TW(tw, p->L1);
+        const dim3 g8(xcd_grid(ch.nB), (unsigned)batch);
+        switch (p->L1) {
+#define C(m)                                                                          \
+  case m:                                                                             \
+    hipLaunchKernelGGL(k_adj_fullB<m>, g8, dim3(kAdjNT4), 0, st, a, ch.d_B, ch.nB,     \
+                       p->d_adj_seg, tw, p->L2, (int)batch, scr);                     \
+    break;
+          C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+          default:
+            ssqk_host::set_error("four-step size %d unsupported", p->L1);
+            return SSQK_ERR_UNSUPPORTED;
+        }
+        SSQK_CHECK(hipGetLastError());
+      }
+    }
+  }
+  float2* S = (float2*)(ws + off[4]);
+  hipLaunchKernelGGL(k_adj_accum, dim3((unsigned)((L + 255) / 256), (unsigned)batch),
+                     dim3(256), 0, st, a, p->d_adj_seg, S);
+  SSQK_CHECK(hipGetLastError());
+  float* xp = (float*)(ws + off[5]);
+  if (L <= kSmallFwd) {
+    SSQK_GET_TW(tw, (int)L);
+    switch ((int)L) {
+#define C(m)                                                                          \
+  case m:                                                                             \
+    hipLaunchKernelGGL(k_adj_ifft_small<m>, dim3((unsigned)batch), dim3(kNT), 0, st, S, \
+                       tw, xp);                                                       \
+    break;
+      SSQK_POW2_CASES(C) C(16384)
+#undef C
+      default:
+        ssqk_host::set_error("adjoint length %lld unsupported", (long long)L);
+        return SSQK_ERR_UNSUPPORTED;
+    }
+    SSQK_CHECK(hipGetLastError());
+  } else {
+    int L1, L2;
+    split_four_step(L, &L1, &L2);
+    float2* scr = S + (size_t)batch * L;
+    {
+      SSQK_GET_TW(tw, L2);
+      const dim3 g(L1 / (kP / L2), (unsigned)batch);
+      switch (L2) {
+#define C(m)                                                                          \
+  case m:                                                                             \
+    hipLaunchKernelGGL(k_adj_ifftA<m>, g, dim3(kNT), 0, st, S, tw, L1, a.ptab, a.pS,   \
+                       scr);                                                          \
+    break;
+        C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096) C(8192)
+#undef C
+        default:
+          ssqk_host::set_error("adjoint length %lld unsupported", (long long)L);
+          return SSQK_ERR_UNSUPPORTED;
+      }
+      SSQK_CHECK(hipGetLastError());
+    }
+    {
+      SSQK_GET_TW(tw, L1);
+      const dim3 g(L2 / (kP / L1), (unsigned)batch);
+      switch (L1) {
+#define C(m)                                                                          \
+  case m:                                                                             \
+    hipLaunchKernelGGL(k_adj_ifftB<m>, g, dim3(kNT), 0, st, scr, tw, L2, xp);          \
+    break;
+        C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096) C(8192)
+#undef C
+        default:
+          ssqk_host::set_error("adjoint length %lld unsupported", (long long)L);
+          return SSQK_ERR_UNSUPPORTED;
+      }
+      SSQK_CHECK(hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(k_adj_fold, dim3((unsigned)((N + 255) / 256), (unsigned)batch),
+                     dim3(256), 0, st, xp, L, N, p->n1, p->pad, gx);
+  SSQK_CHECK(hipGetLastError());
+  return SSQK_OK;
+}
+
+}  // namespace
+
+extern "C" int ssqk_cwt_adjoint_workspace_size(ssqk_cwt_plan p, int64_t batch, int with_gtx,
+                                               size_t* bytes) {
+  if (!p || !bytes || batch < 1) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_adjoint_workspace_size");
+    return SSQK_ERR_ARG;
+  }
+  size_t off[6];
+  adj_ws_layout(p, batch, with_gtx != 0, off, bytes);
+  return SSQK_OK;
+}
+
+extern "C" int ssqk_cwt_adjoint(ssqk_cwt_plan p, const void* gTx, const void* gWx,
+                                const uint16_t* bins, const float* cst, int64_t nf,
+                                int64_t batch, float* gx, void* workspace, void* stream) {
+  if (!p || !gx || !workspace || batch < 1 || batch > 65535 || (!gTx && !gWx) ||
+      (gTx && (!bins || !cst || nf < 1 || nf >= kNoBin))) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_adjoint");
+    return SSQK_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(p->exec_mu);
+  return adjoint_impl(p, (const float2*)gTx, (const float2*)gWx, bins, cst, nf, batch, gx,
+                      workspace, (hipStream_t)stream);
 }
