@@ -79,6 +79,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--N', type=int, default=160000)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-backward', action='store_true',
+                    help='skip the forward+backward timing')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='end with a 256 MiB device copy (counter calibration)')
     args = ap.parse_args()
@@ -152,9 +154,55 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
     plan = list(_lib._plans.values())[-1]
+    # forward + backward of mean(| |Tx| - |Tx_target| |^2) through the
+    # hand-written adjoint (squeeze rows held fixed), target from a second
+    # seeded signal; ratio against the forward step above
+    fb = None
+    if not args.no_backward:
+        tgt = sq.ssq_cwt(torch.as_tensor(chirp_tone_noise(N, 1), device=dev),
+                         wav)[0].abs()
+
+        def fwd_bwd():
+            xg = x_dev.detach().requires_grad_(True)
+            T = sq.ssq_cwt(xg, wav)[0]
+            ((T.abs() - tgt) ** 2).mean().backward()
+            return xg.grad
+
+        for _ in range(2):
+            fwd_bwd()
+        torch.cuda.synchronize()
+        fb_steps = max(args.steps // 2, 3)
+        t0 = time.perf_counter()
+        for _ in range(fb_steps):
+            fwd_bwd()
+        torch.cuda.synchronize()
+        ms_fb = (time.perf_counter() - t0) * 1e3 / fb_steps
+        # the adjoint alone (what the autograd backward of ssq_cwt runs),
+        # on the gradient of the same loss w.r.t. Tx
+        Tq = out[0].detach().requires_grad_(True)
+        ((Tq.abs() - tgt) ** 2).mean().backward()
+        gT = Tq.grad.contiguous()
+        bins_a = bins
+        gx = torch.empty_like(x_dev)
+        plan.adjoint(gx, gTx=gT, bins=bins_a, cst=cst, nf=na)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(fb_steps):
+            plan.adjoint(gx, gTx=gT, bins=bins_a, cst=cst, nf=na)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_adj = e0.elapsed_time(e1) / fb_steps
+        fb = {"ms_per_step": round(ms_fb, 4), "ratio_to_forward": round(ms_fb / ms_step, 3),
+              "adjoint_ms": round(ms_adj, 4),
+              "ratio_forward_plus_adjoint": round((ms_step + ms_adj) / ms_step, 3),
+              "steps": fb_steps,
+              "loss": "mean(||Tx| - |Tx_target||^2), Tx_target from seed 1; gradient "
+                      "w.r.t. x by the adjoint (ssqk_cwt_adjoint); ms_per_step includes "
+                      "the loss and its torch backward"}
     paths = dict(zip(('band', 'overlap_save', 'four_step'), plan.paths()))
     res = {
-        "metric": "ssq_cwt Gscale*samples/s (N=160k, ~300 scales, fp32)",
+        "metric": "ssq_cwt Gscale*samples/s (N=160k, ~300 scales, fp32); "
+                  "fwd+bwd/fwd ratio",
         "value": round(value, 3),
         "unit": "Gscale*samples/s",
         "n_gpus": 1,
@@ -186,6 +234,8 @@ def main():
                                       "Wx and Tx written once (16 B per "
                                       "scale*sample)"}},
     }
+    if fb is not None:
+        res["fwd_bwd"] = fb
     if not args.no_cpu:
         cb = cpu_baseline(x, g)
         res["cpu_baseline"] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind',

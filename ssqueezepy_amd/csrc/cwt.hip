@@ -888,10 +888,13 @@ struct ssqk_cwt_plan_s {
   // and per scale the window {ka, len} of its psih table and spectrum
   struct AdjCls {
     int M = 0, ph = 1;        // band size; phase blocks per workgroup
-    int ntasks = 0, nred = 0;
+    int ntasks = 0;
+    int64_t part_off = 0;     // first partial element of the class
     int4* d_tasks = nullptr;  // {scale, first phase, ka, log2 D}
-    int4* d_red = nullptr;    // {scale, first task, tasks, -}
   };
+  int adj_nred = 0;           // reduce tasks {scale, m0, partials, first partial}
+  int4* d_adj_red = nullptr;
+  int* d_adj_red_M = nullptr;
   std::vector<AdjCls> adj;
   std::vector<int> adj_wide;
   int adj_slots = 0;
@@ -1158,30 +1161,50 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
     }
     p->tab_total = off;
     e = upload(&p->d_adj_seg, seg);
+    std::vector<int4> red;
+    std::vector<int> red_M;
+    int64_t part_total = 0;
     for (int lM = 1; lM < 14 && e == hipSuccess; ++lM) {
       const int M = 1 << lM;
-      std::vector<int4> tasks, red;
+      std::vector<int4> tasks;
       const int64_t D = L / M;
       const int rows = band_rows(M);
-      const int ph = (int)std::max<int64_t>(1, std::min<int64_t>(8, D / rows));
+      const int64_t blocks_per_scale = (D + rows - 1) / rows;
+      int64_t nsc = 0;
+      for (int64_t i = 0; i < na; ++i) nsc += adj_M[i] == M;
+      if (!nsc) continue;
+      // phase blocks per workgroup: up to 8 while >= 1024 workgroups remain
+      const int ph = (int)std::max<int64_t>(
+          1, std::min<int64_t>({8, blocks_per_scale, nsc * blocks_per_scale / 1024}));
       for (int64_t i = 0; i < na; ++i) {
         if (adj_M[i] != M) continue;
         const int t0 = (int)tasks.size();
         for (int64_t a0 = 0; a0 < D; a0 += (int64_t)rows * ph)
           tasks.push_back(make_int4((int)i, (int)a0, band[i].x, ilog2(D)));
-        red.push_back(make_int4((int)i, t0, (int)tasks.size() - t0, 0));
+        const int nb = (int)tasks.size() - t0;
+        for (int m0 = 0; m0 < M; m0 += 256) {
+          red.push_back(make_int4((int)i, m0, nb, (int)(part_total + (int64_t)t0 * M)));
+          red_M.push_back(M);
+        }
       }
-      if (tasks.empty()) continue;
       ssqk_cwt_plan_s::AdjCls c;
       c.M = M;
       c.ph = ph;
       c.ntasks = (int)tasks.size();
-      c.nred = (int)red.size();
+      c.part_off = part_total;
       e = upload(&c.d_tasks, tasks);
-      if (e == hipSuccess) e = upload(&c.d_red, red);
       p->adj.push_back(c);
-      p->adj_partial = std::max<int64_t>(p->adj_partial, (int64_t)c.ntasks * M);
+      part_total += (int64_t)c.ntasks * M;
     }
+    if (part_total >= ((int64_t)1 << 31)) {
+      ssqk_cwt_plan_destroy(p);
+      ssqk_host::set_error("adjoint partial sums too large (%lld)", (long long)part_total);
+      return SSQK_ERR_UNSUPPORTED;
+    }
+    p->adj_partial = part_total;
+    p->adj_nred = (int)red.size();
+    if (e == hipSuccess && !red.empty()) e = upload(&p->d_adj_red, red);
+    if (e == hipSuccess && !red_M.empty()) e = upload(&p->d_adj_red_M, red_M);
     for (int64_t i = 0; i < na; ++i)
       if (!adj_M[i]) p->adj_wide.push_back((int)i);
     if (!p->adj_wide.empty() && e == hipSuccess) {
@@ -1370,10 +1393,9 @@ extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
   (void)hipFree(p->d_psih);
   (void)hipFree(p->d_psih_off);
   (void)hipFree(p->d_coef_band);
-  for (auto& c : p->adj) {
-    (void)hipFree(c.d_tasks);
-    (void)hipFree(c.d_red);
-  }
+  for (auto& c : p->adj) (void)hipFree(c.d_tasks);
+  (void)hipFree(p->d_adj_red);
+  (void)hipFree(p->d_adj_red_M);
   for (auto& c : p->adj_chunks) {
     (void)hipFree(c.d_A);
     (void)hipFree(c.d_B);
@@ -1615,7 +1637,10 @@ struct AdjArgs {
 };
 
 // ybar = gWx + c_i gTx[row, j]: a workgroup holds the gTx columns
-// [j0, j0 + W) of all nf rows in LDS and gathers from them.
+// [j0, j0 + W) of all nf rows in LDS and gathers from them; rows and bins
+// are loaded kYbU iterations ahead (loads out of branches).
+constexpr int kYbU = 8;
+
 template <int W>
 __global__ __launch_bounds__(256) void k_adj_ybar(const float2* __restrict__ gTx,
                                                   const float2* __restrict__ gWx,
@@ -1626,27 +1651,52 @@ __global__ __launch_bounds__(256) void k_adj_ybar(const float2* __restrict__ gTx
   const int b = blockIdx.y;
   const int64_t j0 = (int64_t)blockIdx.x * W;
   const float2* tx = gTx + (int64_t)b * nf * N;
-  for (int e = threadIdx.x; e < nf * W; e += 256) {
-    const int k = e / W, c = e - k * W;
-    const int64_t j = j0 + c;
-    s_tx[e] = tx[(int64_t)k * N + (j < N ? j : N - 1)];
+  const int tot_t = nf * W;
+  for (int e0 = 0; e0 < tot_t; e0 += 256 * kYbU) {
+    float2 v[kYbU];
+#pragma unroll
+    for (int u = 0; u < kYbU; ++u) {
+      const int e = min(e0 + threadIdx.x + u * 256, tot_t - 1);
+      const int k = e / W, c = e - k * W;
+      v[u] = tx[(int64_t)k * N + min(j0 + c, N - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < kYbU; ++u) {
+      const int e = e0 + threadIdx.x + u * 256;
+      if (e < tot_t) s_tx[e] = v[u];
+    }
   }
   __syncthreads();
   const int64_t base = (int64_t)b * na * N;
-  for (int e = threadIdx.x; e < na * W; e += 256) {
-    const int i = e / W, c = e - i * W;
-    const int64_t j = j0 + c;
-    if (j >= N) continue;
-    const int64_t o = base + (int64_t)i * N + j;
-    float2 v = gWx ? gWx[o] : make_float2(0.f, 0.f);
-    const int k = bins[o];
-    if (k != kNoBin && k < nf) {
-      const float2 t = s_tx[k * W + c];
-      const float ci = cst[i];
-      v.x += ci * t.x;
-      v.y += ci * t.y;
+  const int tot = na * W;
+  for (int e0 = 0; e0 < tot; e0 += 256 * kYbU) {
+    int kk[kYbU];
+    float2 gw[kYbU];
+    float cc[kYbU];
+#pragma unroll
+    for (int u = 0; u < kYbU; ++u) {
+      const int e = min(e0 + threadIdx.x + u * 256, tot - 1);
+      const int i = e / W, c = e - i * W;
+      const int64_t o = base + (int64_t)i * N + min(j0 + c, N - 1);
+      kk[u] = bins[o];
+      gw[u] = gWx ? gWx[o] : make_float2(0.f, 0.f);
+      cc[u] = cst[i];
     }
-    yb[o] = v;
+#pragma unroll
+    for (int u = 0; u < kYbU; ++u) {
+      const int e = e0 + threadIdx.x + u * 256;
+      const int i = e / W, c = e - i * W;
+      const int64_t j = j0 + c;
+      const int k = kk[u];
+      const bool hit = k != kNoBin && k < nf;
+      const float2 t = s_tx[(hit ? k : 0) * W + c];
+      float2 v = gw[u];
+      if (hit) {
+        v.x += cc[u] * t.x;
+        v.y += cc[u] * t.y;
+      }
+      if (e < tot && j < N) yb[base + (int64_t)i * N + j] = v;
+    }
   }
 }
 
@@ -1659,6 +1709,7 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
                                                              const int4* __restrict__ tasks,
                                                              int ntasks, int ph,
                                                              const float2* __restrict__ tw,
+                                                             int64_t part_stride,
                                                              float2* __restrict__ partial) {
   constexpr int ROWS = band_rows(M);
   constexpr int NT = band_threads(M);
@@ -1710,7 +1761,7 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
     }
     __syncthreads();
   }
-  float2* out = partial + ((int64_t)b * ntasks + task) * M;
+  float2* out = partial + (int64_t)b * part_stride + (int64_t)task * M;
   if constexpr (ROWS == 1) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) out[threadIdx.x + it * NT] = acc[it];
@@ -1735,23 +1786,26 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
   }
 }
 
-// Y_i[m] = sum of scale i's partials (its tasks in order).
+// Y_i[m] for 256 consecutive m of one scale: the sum of its partials in
+// task order.  task = {scale, m0, tasks, first partial (elements, this
+// signal's partial buffer)}; one launch for every band class.
 __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a, const int4* __restrict__ red,
-                                                    int M, int ntasks,
+                                                    const int* __restrict__ red_M,
+                                                    int64_t part_stride,
                                                     const float2* __restrict__ partial) {
   const int4 r = red[blockIdx.x];
+  const int M = red_M[blockIdx.x];
   const int b = blockIdx.y;
-  const float2* pb = partial + ((int64_t)b * ntasks + r.y) * M;
-  float2* y = a.ytab + (int64_t)b * a.ytab_stride + a.psih_off[r.x];
-  for (int m = threadIdx.x; m < M; m += 256) {
-    float2 v = make_float2(0.f, 0.f);
-    for (int t = 0; t < r.z; ++t) {
-      const float2 q = pb[(int64_t)t * M + m];
-      v.x += q.x;
-      v.y += q.y;
-    }
-    y[m] = v;
+  const int m = r.y + threadIdx.x;
+  if (m >= M) return;
+  const float2* pb = partial + (int64_t)b * part_stride + (int64_t)(uint32_t)r.w;
+  float2 v = make_float2(0.f, 0.f);
+  for (int t = 0; t < r.z; ++t) {
+    const float2 q = pb[(int64_t)t * M + m];
+    v.x += q.x;
+    v.y += q.y;
   }
+  a.ytab[(int64_t)b * a.ytab_stride + a.psih_off[r.x] + m] = v;
 }
 
 // Four-step forward FFT of u = ybar_i at the padded positions for the
@@ -1949,14 +2003,14 @@ __global__ __launch_bounds__(256) void k_adj_fold(const float* __restrict__ xp, 
 }
 
 int launch_adj_band(int M, dim3 grid, hipStream_t st, const AdjArgs& a, const int4* tasks,
-                    int ph, float2* partial) {
+                    int ph, int64_t part_stride, float2* partial) {
   SSQK_GET_TW(tw, M);
   const dim3 g8(xcd_grid(grid.x), grid.y);
   switch (M) {
 #define C(m)                                                                        \
   case m:                                                                           \
     hipLaunchKernelGGL(k_adj_band<m>, g8, dim3(band_threads(m)), 0, st, a, tasks,   \
-                       (int)grid.x, ph, tw, partial);                               \
+                       (int)grid.x, ph, tw, part_stride, partial);                  \
     break;
     SSQK_POW2_CASES(C)
 #undef C
@@ -2041,11 +2095,13 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
   float2* partial = (float2*)(ws + off[2]);
   for (const auto& c : p->adj) {
     int rc = launch_adj_band(c.M, dim3(c.ntasks, (unsigned)batch), st, a, c.d_tasks, c.ph,
-                             partial);
+                             p->adj_partial, partial + c.part_off);
     if (rc) return rc;
     SSQK_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_adj_reduce, dim3(c.nred, (unsigned)batch), dim3(256), 0, st, a,
-                       c.d_red, c.M, c.ntasks, partial);
+  }
+  if (p->adj_nred) {
+    hipLaunchKernelGGL(k_adj_reduce, dim3(p->adj_nred, (unsigned)batch), dim3(256), 0, st, a,
+                       p->d_adj_red, p->d_adj_red_M, p->adj_partial, partial);
     SSQK_CHECK(hipGetLastError());
   }
   if (!p->adj_chunks.empty()) {
