@@ -892,6 +892,11 @@ struct ssqk_cwt_plan_s {
     int64_t part_off = 0;     // first partial element of the class
     int4* d_tasks = nullptr;  // {scale, first phase, ka, log2 D}
   };
+  int adj_os_ngroups = 0;     // overlap-save adjoint: {F, B, H, segments} per group
+  int4* d_adj_os_grp = nullptr;
+  int64_t* d_adj_os_base = nullptr;
+  std::vector<int64_t> adj_os_cls_off;   // first row element of each class's tasks
+  int64_t adj_os_rows = 0;    // floats per signal
   int adj_nred = 0;           // reduce tasks {scale, m0, partials, first partial}
   int4* d_adj_red = nullptr;
   int* d_adj_red_M = nullptr;
@@ -1062,13 +1067,14 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
         if (c < cost) { cost = c; bestF = F; bestH = H; }
       }
     }
-    tab_len[i] = M <= kP ? M : K;            // psih window (also the adjoint's)
-    adj_M[i] = M <= kP ? (int)M : 0;
-    if (bestF) {
+    if (bestF) {                             // overlap-save (forward and adjoint)
       os_F[i] = bestF;
       os_H[i] = bestH;
+      adj_M[i] = -1;
       continue;
     }
+    tab_len[i] = M <= kP ? M : K;            // psih window (also the adjoint's)
+    adj_M[i] = M <= kP ? (int)M : 0;
     if (M <= kP) {
       const int lM = ilog2(M);
       const int64_t D = L / M;
@@ -1206,7 +1212,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
     if (e == hipSuccess && !red.empty()) e = upload(&p->d_adj_red, red);
     if (e == hipSuccess && !red_M.empty()) e = upload(&p->d_adj_red_M, red_M);
     for (int64_t i = 0; i < na; ++i)
-      if (!adj_M[i]) p->adj_wide.push_back((int)i);
+      if (adj_M[i] == 0) p->adj_wide.push_back((int)i);
     if (!p->adj_wide.empty() && e == hipSuccess) {
       if (!p->L1) split_four_step(L, &p->L1, &p->L2);
       const int64_t per_slot = L * (int64_t)sizeof(float2);
@@ -1295,6 +1301,30 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       p->coef_stride = 2 * Fmax;
       e = hipMalloc(&p->d_coef, sizeof(float2) * (size_t)p->n_os * p->coef_stride);
       if (e == hipSuccess) e = upload(&p->d_coef_band, coef_band);
+      // overlap-save adjoint: one row of F floats per task, classes in order
+      std::vector<int4> ag;
+      std::vector<int64_t> ab;
+      int64_t rows = 0;
+      size_t gfirst = 0;
+      for (size_t c = 0; c < p->os.size(); ++c) {
+        p->adj_os_cls_off.push_back(rows);
+        const int F = p->os[c].F;
+        const size_t ng = grp_by_F[c].size();
+        int64_t task0 = 0;
+        for (size_t gi = 0; gi < ng; ++gi) {
+          const OsGroup& g = grp_by_F[c][gi];
+          const int64_t nblk = (N + g.B - 1) / g.B;
+          ag.push_back(make_int4(F, g.B, g.H, (int)nblk));
+          ab.push_back(rows + task0 * F);
+          task0 += nblk;
+        }
+        rows += task0 * F;
+        gfirst += ng;
+      }
+      p->adj_os_ngroups = (int)ag.size();
+      p->adj_os_rows = rows;
+      if (e == hipSuccess) e = upload(&p->d_adj_os_grp, ag);
+      if (e == hipSuccess) e = upload(&p->d_adj_os_base, ab);
     }
   }
   if (const char* ev = std::getenv("SSQK_SIDE_STREAMS"))
@@ -1396,6 +1426,8 @@ extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
   for (auto& c : p->adj) (void)hipFree(c.d_tasks);
   (void)hipFree(p->d_adj_red);
   (void)hipFree(p->d_adj_red_M);
+  (void)hipFree(p->d_adj_os_grp);
+  (void)hipFree(p->d_adj_os_base);
   for (auto& c : p->adj_chunks) {
     (void)hipFree(c.d_A);
     (void)hipFree(c.d_B);
@@ -1808,6 +1840,132 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a, const int4* __res
   a.ytab[(int64_t)b * a.ytab_stride + a.psih_off[r.x] + m] = v;
 }
 
+// Overlap-save adjoint: for one segment of one group, the correlation of
+// each scale's ybar segment with its impulse response, summed over the
+// group's scales in the frequency domain:
+//   xloc = ifft_F( sum_s conj(coef_s) . fft_F(ybar_s at local [H, H + B)) )
+// (coef_s = F-point spectrum of h_s / F, as in the forward).  The F real
+// values cover padded positions [n1 + blk B - H, + F) modulo L; they go to
+// a per-task row, summed per position by k_adj_os_add.
+template <int F>
+struct AdjOsShape {
+  static constexpr int SR = OsShape<F>::SR;                    // rows per FFT batch
+  static constexpr int NT = (SR * F) / 16 < 64 ? 64 : (SR * F) / 16;
+  static constexpr int KT = (F + NT - 1) / NT;                  // bins per thread
+};
+
+template <int F>
+__global__ __launch_bounds__(AdjOsShape<F>::NT) void k_adj_os(AdjArgs a,
+                                                              const OsGroup* __restrict__ groups,
+                                                              const int2* __restrict__ tasks,
+                                                              int ntasks,
+                                                              const float2* __restrict__ coef,
+                                                              int coef_stride,
+                                                              const int2* __restrict__ coef_band,
+                                                              const float2* __restrict__ tw,
+                                                              float* __restrict__ out,
+                                                              int64_t out_stride) {
+  constexpr int SR = AdjOsShape<F>::SR;
+  constexpr int NT = AdjOsShape<F>::NT;
+  constexpr int KT = AdjOsShape<F>::KT;
+  constexpr int IC = (SR * F + NT - 1) / NT;
+  __shared__ float2 s[Lds<F>::elems(SR)];
+  const int task = blockIdx.x;
+  const int2 t = tasks[task];
+  const OsGroup g = groups[t.x];
+  const int b = blockIdx.y;
+  const int64_t j_base = (int64_t)t.y * g.B;
+  float2 acc[KT];
+#pragma unroll
+  for (int u = 0; u < KT; ++u) acc[u] = make_float2(0.f, 0.f);
+  for (int s0 = 0; s0 < g.ns; s0 += SR) {
+    int tx = threadIdx.x;
+    asm volatile("" : "+v"(tx));
+    float2 v[IC];
+#pragma unroll
+    for (int it = 0; it < IC; ++it) {
+      const int e = tx + it * NT;
+      const int r = e / F, q = e - r * F;
+      const int sc = g.i0 + min(s0 + r, g.ns - 1);
+      const int64_t j = j_base + q - g.H;
+      const float2* yb = a.yb + (int64_t)b * a.yb_stride + (int64_t)sc * a.N;
+      v[it] = yb[(j >= 0 && j < a.N) ? j : 0];
+    }
+#pragma unroll
+    for (int it = 0; it < IC; ++it) {
+      const int e = tx + it * NT;
+      const int r = e / F, q = e - r * F;
+      const int64_t j = j_base + q - g.H;
+      const bool in = e < SR * F && s0 + r < g.ns && q >= g.H && q < g.H + g.B && j < a.N;
+      if (e < SR * F) s[Lds<F>::idx(r, q)] = in ? v[it] : make_float2(0.f, 0.f);
+    }
+    float2* const bufs[1] = {s};
+    lds_fft<F, SR, -1, NT, 1>(bufs, tw);
+#pragma unroll
+    for (int u = 0; u < KT; ++u) {
+      const int k = tx + u * NT;
+      if (k < F) {
+        for (int r = 0; r < SR && s0 + r < g.ns; ++r) {
+          const int rank = g.rank + s0 + r;
+          const int2 kb = coef_band[rank];
+          const int kc = min(max(k, kb.x), kb.y);
+          const float2 c = coef[(int64_t)rank * coef_stride + kc];
+          if (k == kc) {
+            const float2 y = cmul(s[Lds<F>::idx(r, k)], cconj(c));
+            acc[u].x += y.x;
+            acc[u].y += y.y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < KT; ++u) {
+    const int k = threadIdx.x + u * NT;
+    if (k < F) s[Lds<F>::idx(0, k)] = acc[u];
+  }
+  {
+    float2* const bufs[1] = {s};
+    lds_fft<F, 1, +1, NT, 1>(bufs, tw);
+  }
+  float* o = out + (int64_t)b * out_stride + (int64_t)task * F;
+  for (int e = threadIdx.x; e < F; e += NT) o[e] = s[Lds<F>::idx(0, e)].x;
+}
+
+// xp[t] += the overlap-save adjoint rows covering padded position t, groups
+// and segments in order.  grp = {F, B, H, nblk}, grp_base[g] = first row
+// element of the group's tasks.
+__global__ __launch_bounds__(256) void k_adj_os_add(const int4* __restrict__ grp,
+                                                    const int64_t* __restrict__ grp_base,
+                                                    int ngroups, int64_t L, int64_t n1,
+                                                    const float* __restrict__ rows,
+                                                    int64_t rows_stride,
+                                                    float* __restrict__ xp) {
+  const int b = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= L) return;
+  const float* rb = rows + (int64_t)b * rows_stride;
+  float acc = 0.f;
+  for (int gi = 0; gi < ngroups; ++gi) {
+    const int4 q = grp[gi];
+    const int64_t F = q.x, B = q.y, H = q.z, nblk = q.w;
+    const int64_t span = (nblk - 1) * B + F;
+    int64_t u = (t - (n1 - H)) % L;
+    if (u < 0) u += L;
+    for (; u < span; u += L) {
+      int64_t bk_hi = u / B;
+      if (bk_hi > nblk - 1) bk_hi = nblk - 1;
+      int64_t bk_lo = u - F + 1 <= 0 ? 0 : (u - F + 1 + B - 1) / B;
+      for (int64_t bk = bk_lo; bk <= bk_hi; ++bk) {
+        const int64_t e = u - bk * B;
+        acc += rb[grp_base[gi] + bk * F + e];
+      }
+    }
+  }
+  xp[(int64_t)b * L + t] += acc;
+}
+
 // Four-step forward FFT of u = ybar_i at the padded positions for the
 // scales with bands wider than kP; band bins go to the spectrum table.
 constexpr int kAdjNT4 = kPF / 16;             // one radix-16 butterfly per thread
@@ -2021,7 +2179,7 @@ int launch_adj_band(int M, dim3 grid, hipStream_t st, const AdjArgs& a, const in
   return SSQK_OK;
 }
 
-void adj_ws_layout(const ssqk_cwt_plan_s* p, int64_t batch, bool need_yb, size_t off[6],
+void adj_ws_layout(const ssqk_cwt_plan_s* p, int64_t batch, bool need_yb, size_t off[7],
                    size_t* total) {
   auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
   size_t o = 0;
@@ -2037,6 +2195,8 @@ void adj_ws_layout(const ssqk_cwt_plan_s* p, int64_t batch, bool need_yb, size_t
   o = al(o + sizeof(float2) * (size_t)(2 * batch * p->L));
   off[5] = o;                                        // xp
   o = al(o + sizeof(float) * (size_t)(batch * p->L));
+  off[6] = o;                                        // overlap-save adjoint rows
+  o = al(o + sizeof(float) * (size_t)(batch * std::max<int64_t>(p->adj_os_rows, 1)));
   *total = o;
 }
 
@@ -2044,7 +2204,7 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
                  const uint16_t* bins, const float* cst, int64_t nf, int64_t batch,
                  float* gx, void* workspace, hipStream_t st) {
   const bool need_yb = gTx != nullptr;
-  size_t off[6], total;
+  size_t off[7], total;
   adj_ws_layout(p, batch, need_yb, off, &total);
   char* ws = (char*)workspace;
   const int64_t N = p->N, L = p->L;
@@ -2143,6 +2303,27 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
       }
     }
   }
+  float* osrows = (float*)(ws + off[6]);
+  for (size_t c = 0; c < p->os.size(); ++c) {
+    const auto& oc = p->os[c];
+    float* o = osrows + p->adj_os_cls_off[c];
+    SSQK_GET_TW(tw, oc.F);
+    switch (oc.F) {
+#define C(m)                                                                            \
+  case m:                                                                               \
+    hipLaunchKernelGGL(k_adj_os<m>, dim3(oc.ntasks, (unsigned)batch),                   \
+                       dim3(AdjOsShape<m>::NT), 0, st, a, oc.d_groups, oc.d_tasks,       \
+                       oc.ntasks, p->d_coef, p->coef_stride, p->d_coef_band, tw, o,     \
+                       std::max<int64_t>(p->adj_os_rows, 1));                           \
+    break;
+      SSQK_OS_CASES(C)
+#undef C
+      default:
+        ssqk_host::set_error("overlap-save size %d unsupported", oc.F);
+        return SSQK_ERR_UNSUPPORTED;
+    }
+    SSQK_CHECK(hipGetLastError());
+  }
   float2* S = (float2*)(ws + off[4]);
   hipLaunchKernelGGL(k_adj_accum, dim3((unsigned)((L + 255) / 256), (unsigned)batch),
                      dim3(256), 0, st, a, p->d_adj_seg, S);
@@ -2201,6 +2382,12 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
       SSQK_CHECK(hipGetLastError());
     }
   }
+  if (p->adj_os_ngroups) {
+    hipLaunchKernelGGL(k_adj_os_add, dim3((unsigned)((L + 255) / 256), (unsigned)batch),
+                       dim3(256), 0, st, p->d_adj_os_grp, p->d_adj_os_base, p->adj_os_ngroups,
+                       L, p->n1, osrows, std::max<int64_t>(p->adj_os_rows, 1), xp);
+    SSQK_CHECK(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_adj_fold, dim3((unsigned)((N + 255) / 256), (unsigned)batch),
                      dim3(256), 0, st, xp, L, N, p->n1, p->pad, gx);
   SSQK_CHECK(hipGetLastError());
@@ -2215,7 +2402,7 @@ extern "C" int ssqk_cwt_adjoint_workspace_size(ssqk_cwt_plan p, int64_t batch, i
     ssqk_host::set_error("bad arguments to ssqk_cwt_adjoint_workspace_size");
     return SSQK_ERR_ARG;
   }
-  size_t off[6];
+  size_t off[7];
   adj_ws_layout(p, batch, with_gtx != 0, off, bytes);
   return SSQK_OK;
 }
