@@ -905,6 +905,7 @@ struct ssqk_cwt_plan_s {
   int adj_slots = 0;
   std::vector<Chunk> adj_chunks;
   int2* d_adj_seg = nullptr;
+  int2* d_adj_krange = nullptr;   // per 256 bins: first / last scale whose window meets them
   int64_t tab_total = 0;
   int64_t adj_partial = 0;    // complex elements of the largest class's partial sums
   static constexpr int kSide = 3;
@@ -1167,6 +1168,20 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
     }
     p->tab_total = off;
     e = upload(&p->d_adj_seg, seg);
+    {
+      const int64_t nblk = (L + 255) / 256;
+      std::vector<int2> kr(nblk, make_int2(0, -1));
+      for (int64_t i = 0; i < na; ++i) {
+        if (seg[i].y <= 0) continue;
+        const int64_t b0 = seg[i].x / 256, b1 = (seg[i].x + (int64_t)seg[i].y - 1) / 256;
+        for (int64_t bb = b0; bb <= b1 && bb < nblk; ++bb) {
+          if (kr[bb].y < kr[bb].x) kr[bb] = make_int2((int)i, (int)i);
+          kr[bb].x = std::min(kr[bb].x, (int)i);
+          kr[bb].y = std::max(kr[bb].y, (int)i);
+        }
+      }
+      if (e == hipSuccess) e = upload(&p->d_adj_krange, kr);
+    }
     std::vector<int4> red;
     std::vector<int> red_M;
     int64_t part_total = 0;
@@ -1433,6 +1448,7 @@ extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
     (void)hipFree(c.d_B);
   }
   (void)hipFree(p->d_adj_seg);
+  (void)hipFree(p->d_adj_krange);
   for (int k = 0; k < ssqk_cwt_plan_s::kSide; ++k) {
     if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
     if (p->ev_done[k]) (void)hipEventDestroy(p->ev_done[k]);
@@ -2032,6 +2048,7 @@ __global__ __launch_bounds__(kAdjNT4) void k_adj_fullB(AdjArgs a, const int4* __
 // S[k] = sum_i psih_i[k] Y_i[k] over the scales whose window holds k, in
 // ascending scale order; zero above the Nyquist bin.
 __global__ __launch_bounds__(256) void k_adj_accum(AdjArgs a, const int2* __restrict__ seg,
+                                                   const int2* __restrict__ krange,
                                                    float2* __restrict__ S) {
   const int b = blockIdx.y;
   const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -2039,7 +2056,8 @@ __global__ __launch_bounds__(256) void k_adj_accum(AdjArgs a, const int2* __rest
   float2 acc = make_float2(0.f, 0.f);
   if (k <= a.L / 2) {
     const float2* yt = a.ytab + (int64_t)b * a.ytab_stride;
-    for (int i = 0; i < a.na; ++i) {
+    const int2 rg = krange[blockIdx.x];      // scales whose windows meet this block
+    for (int i = rg.x; i <= rg.y; ++i) {
       const int2 sg = seg[i];
       const int64_t m = k - sg.x;
       if (m >= 0 && m < sg.y) {
@@ -2252,81 +2270,125 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
                                                              : go(k_adj_ybar<8>, 2);
     if (rc) return rc;
   }
+  // band classes, the four-step chunks and the overlap-save classes are
+  // independent: spread them over the side streams like the forward
   float2* partial = (float2*)(ws + off[2]);
-  for (const auto& c : p->adj) {
-    int rc = launch_adj_band(c.M, dim3(c.ntasks, (unsigned)batch), st, a, c.d_tasks, c.ph,
-                             p->adj_partial, partial + c.part_off);
-    if (rc) return rc;
-    SSQK_CHECK(hipGetLastError());
+  float* osrows = (float*)(ws + off[6]);
+  struct Item {
+    int kind, idx;
+    double cost;
+  };
+  std::vector<Item> items;
+  for (size_t c = 0; c < p->adj.size(); ++c) {
+    const auto& k = p->adj[c];
+    items.push_back({0, (int)c, (double)k.ntasks * k.ph * band_rows(k.M) * k.M *
+                                    (std::log2((double)k.M) + 4)});
+  }
+  if (!p->adj_chunks.empty())
+    items.push_back({1, 0, (double)p->adj_wide.size() * L * (std::log2((double)L) + 6)});
+  for (size_t c = 0; c < p->os.size(); ++c) {
+    const auto& o = p->os[c];
+    items.push_back({2, (int)c, (double)o.nscales * N * (std::log2((double)o.F) + 3) * o.F /
+                                    (double)std::max(o.F - 2 * o.Hmax, 1) * 0.6});
+  }
+  std::sort(items.begin(), items.end(),
+            [](const Item& u, const Item& v) { return u.cost > v.cost; });
+  constexpr int NS = ssqk_cwt_plan_s::kSide + 1;
+  hipStream_t streams[NS] = {st, p->side[0], p->side[1], p->side[2]};
+  double load[NS] = {0, 0, 0, 0};
+  std::vector<int> where(items.size());
+  for (size_t t = 0; t < items.size(); ++t) {
+    int best = 0;
+    for (int k = 1; k <= p->nside; ++k)
+      if (load[k] < load[best]) best = k;
+    where[t] = best;
+    load[best] += items[t].cost;
+  }
+  bool used[NS] = {true, false, false, false};
+  for (size_t t = 0; t < items.size(); ++t) used[where[t]] = true;
+  SSQK_CHECK(hipEventRecord(p->ev_start, st));
+  for (int k = 1; k < NS; ++k)
+    if (used[k]) SSQK_CHECK(hipStreamWaitEvent(streams[k], p->ev_start, 0));
+  for (size_t t = 0; t < items.size(); ++t) {
+    hipStream_t sk = streams[where[t]];
+    if (items[t].kind == 0) {
+      const auto& c = p->adj[items[t].idx];
+      int rc = launch_adj_band(c.M, dim3(c.ntasks, (unsigned)batch), sk, a, c.d_tasks, c.ph,
+                               p->adj_partial, partial + c.part_off);
+      if (rc) return rc;
+      SSQK_CHECK(hipGetLastError());
+    } else if (items[t].kind == 1) {
+      float2* scr = (float2*)(ws + off[3]);
+      for (const auto& ch : p->adj_chunks) {
+        {
+          SSQK_GET_TW(tw, p->L2);
+          const dim3 g8(xcd_grid(ch.nA), (unsigned)batch);
+          switch (p->L2) {
+#define C(m)                                                                          \
+  case m:                                                                             \
+    hipLaunchKernelGGL(k_adj_fullA<m>, g8, dim3(kAdjNT4), 0, sk, a, ch.d_A, ch.nA, tw, \
+                       p->L1, (int)batch, scr);                                       \
+    break;
+            C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+            default:
+              ssqk_host::set_error("four-step size %d unsupported", p->L2);
+              return SSQK_ERR_UNSUPPORTED;
+          }
+          SSQK_CHECK(hipGetLastError());
+        }
+        {
+          SSQK_GET_TW(tw, p->L1);
+          const dim3 g8(xcd_grid(ch.nB), (unsigned)batch);
+          switch (p->L1) {
+#define C(m)                                                                          \
+  case m:                                                                             \
+    hipLaunchKernelGGL(k_adj_fullB<m>, g8, dim3(kAdjNT4), 0, sk, a, ch.d_B, ch.nB,     \
+                       p->d_adj_seg, tw, p->L2, (int)batch, scr);                     \
+    break;
+            C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+            default:
+              ssqk_host::set_error("four-step size %d unsupported", p->L1);
+              return SSQK_ERR_UNSUPPORTED;
+          }
+          SSQK_CHECK(hipGetLastError());
+        }
+      }
+    } else {
+      const auto& oc = p->os[items[t].idx];
+      float* o = osrows + p->adj_os_cls_off[items[t].idx];
+      SSQK_GET_TW(tw, oc.F);
+      switch (oc.F) {
+#define C(m)                                                                            \
+  case m:                                                                               \
+    hipLaunchKernelGGL(k_adj_os<m>, dim3(oc.ntasks, (unsigned)batch),                   \
+                       dim3(AdjOsShape<m>::NT), 0, sk, a, oc.d_groups, oc.d_tasks,       \
+                       oc.ntasks, p->d_coef, p->coef_stride, p->d_coef_band, tw, o,     \
+                       std::max<int64_t>(p->adj_os_rows, 1));                           \
+    break;
+        SSQK_OS_CASES(C)
+#undef C
+        default:
+          ssqk_host::set_error("overlap-save size %d unsupported", oc.F);
+          return SSQK_ERR_UNSUPPORTED;
+      }
+      SSQK_CHECK(hipGetLastError());
+    }
+  }
+  for (int k = 1; k < NS; ++k) {
+    if (!used[k]) continue;
+    SSQK_CHECK(hipEventRecord(p->ev_done[k - 1], streams[k]));
+    SSQK_CHECK(hipStreamWaitEvent(st, p->ev_done[k - 1], 0));
   }
   if (p->adj_nred) {
     hipLaunchKernelGGL(k_adj_reduce, dim3(p->adj_nred, (unsigned)batch), dim3(256), 0, st, a,
                        p->d_adj_red, p->d_adj_red_M, p->adj_partial, partial);
     SSQK_CHECK(hipGetLastError());
   }
-  if (!p->adj_chunks.empty()) {
-    float2* scr = (float2*)(ws + off[3]);
-    for (const auto& ch : p->adj_chunks) {
-      {
-        SSQK_GET_TW(tw, p->L2);
-        const dim3 g8(xcd_grid(ch.nA), (unsigned)batch);
-        switch (p->L2) {
-#define C(m)                                                                          \
-  case m:                                                                             \
-    hipLaunchKernelGGL(k_adj_fullA<m>, g8, dim3(kAdjNT4), 0, st, a, ch.d_A, ch.nA, tw, \
-                       p->L1, (int)batch, scr);                                       \
-    break;
-          C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
-#undef C
-          default:
-            ssqk_host::set_error("four-step size %d unsupported", p->L2);
-            return SSQK_ERR_UNSUPPORTED;
-        }
-        SSQK_CHECK(hipGetLastError());
-      }
-      {
-        SSQK_GET_TW(tw, p->L1);
-        const dim3 g8(xcd_grid(ch.nB), (unsigned)batch);
-        switch (p->L1) {
-#define C(m)                                                                          \
-  case m:                                                                             \
-    hipLaunchKernelGGL(k_adj_fullB<m>, g8, dim3(kAdjNT4), 0, st, a, ch.d_B, ch.nB,     \
-                       p->d_adj_seg, tw, p->L2, (int)batch, scr);                     \
-    break;
-          C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
-#undef C
-          default:
-            ssqk_host::set_error("four-step size %d unsupported", p->L1);
-            return SSQK_ERR_UNSUPPORTED;
-        }
-        SSQK_CHECK(hipGetLastError());
-      }
-    }
-  }
-  float* osrows = (float*)(ws + off[6]);
-  for (size_t c = 0; c < p->os.size(); ++c) {
-    const auto& oc = p->os[c];
-    float* o = osrows + p->adj_os_cls_off[c];
-    SSQK_GET_TW(tw, oc.F);
-    switch (oc.F) {
-#define C(m)                                                                            \
-  case m:                                                                               \
-    hipLaunchKernelGGL(k_adj_os<m>, dim3(oc.ntasks, (unsigned)batch),                   \
-                       dim3(AdjOsShape<m>::NT), 0, st, a, oc.d_groups, oc.d_tasks,       \
-                       oc.ntasks, p->d_coef, p->coef_stride, p->d_coef_band, tw, o,     \
-                       std::max<int64_t>(p->adj_os_rows, 1));                           \
-    break;
-      SSQK_OS_CASES(C)
-#undef C
-      default:
-        ssqk_host::set_error("overlap-save size %d unsupported", oc.F);
-        return SSQK_ERR_UNSUPPORTED;
-    }
-    SSQK_CHECK(hipGetLastError());
-  }
   float2* S = (float2*)(ws + off[4]);
   hipLaunchKernelGGL(k_adj_accum, dim3((unsigned)((L + 255) / 256), (unsigned)batch),
-                     dim3(256), 0, st, a, p->d_adj_seg, S);
+                     dim3(256), 0, st, a, p->d_adj_seg, p->d_adj_krange, S);
   SSQK_CHECK(hipGetLastError());
   float* xp = (float*)(ws + off[5]);
   if (L <= kSmallFwd) {
