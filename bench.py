@@ -40,6 +40,60 @@ def chirp_tone_noise(N, seed):
     return x.astype(np.float32)
 
 
+def am_fm_chirp(N, rng):
+    """AM/FM chirp with carrier inside [0.01, 0.4] cycles/sample, AM depth
+    U[0, 0.5], Gaussian noise at SNR U[0, 20] dB (float32)."""
+    n = np.arange(N, dtype=np.float64)
+    f0, f1 = np.sort(rng.uniform(0.01, 0.4, 2))
+    phase = 2 * np.pi * (f0 * n + 0.5 * (f1 - f0) * n**2 / N)
+    depth = rng.uniform(0, 0.5)
+    am = 1 + depth * np.cos(2 * np.pi * rng.uniform(1, 8) * n / N)
+    x = am * np.cos(phase)
+    snr = rng.uniform(0, 20)
+    x = x + rng.standard_normal(N) * np.sqrt(np.var(x) / 10**(snr / 10))
+    return x.astype(np.float32)
+
+
+def batched_uneven(torch, sq, n_signals=256, passes=3):
+    """256 signals, lengths U[4096, 65536], Morlet mu=6, nv=32: one
+    ssq_cwt_batch pass (all signals queued on 4 streams) against a loop of
+    per-signal ssq_cwt calls, both with plans and grids cached."""
+    from ssqueezepy_amd import _lib
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+    _lib.PLAN_CACHE = max(_lib.PLAN_CACHE, 2 * n_signals)
+    rng = np.random.default_rng(3)
+    lengths = rng.integers(4096, 65537, n_signals)
+    dev = torch.device('cuda', 0)
+    xs = [torch.as_tensor(am_fm_chirp(int(n), rng), device=dev) for n in lengths]
+    wav = sq.Wavelet(('morlet', {'mu': 6.}))
+    units = sum(ssq_cwt_grid(int(n), wav)['na'] * int(n) for n in lengths)
+
+    def loop():
+        for x in xs:
+            sq.ssq_cwt(x, wav)
+
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        loop()
+    torch.cuda.synchronize()
+    ms_loop = (time.perf_counter() - t0) * 1e3 / passes
+    sq.ssq_cwt_batch(xs, wav)                 # builds the runner (plans, buffers)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        sq.ssq_cwt_batch(xs, wav)
+    torch.cuda.synchronize()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / passes
+    return {"workload": "%d signals, N ~ U[4096, 65536] (seed 3), AM/FM chirp + noise "
+                        "(SNR U[0,20] dB), Morlet mu=6, nv=32" % n_signals,
+            "scale_samples": int(units), "ms_per_signal_loop": round(ms_loop, 3),
+            "ms_batched": round(ms_batch, 3), "speedup": round(ms_loop / ms_batch, 2),
+            "value_batched": round(units / ms_batch / 1e6, 3),
+            "unit": "Gscale*samples/s"}
+
+
 def _traffic():
     """HBM bytes per k_squeeze launch measured with rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy), kept in
@@ -79,6 +133,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--N', type=int, default=160000)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-batched', action='store_true',
+                    help='skip the uneven-length batch comparison')
     ap.add_argument('--no-backward', action='store_true',
                     help='skip the forward+backward timing')
     ap.add_argument('--copy-calibration', action='store_true',
@@ -236,6 +292,8 @@ def main():
     }
     if fb is not None:
         res["fwd_bwd"] = fb
+    if not args.no_batched:
+        res["batched_uneven"] = batched_uneven(torch, sq)
     if not args.no_cpu:
         cb = cpu_baseline(x, g)
         res["cpu_baseline"] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind',

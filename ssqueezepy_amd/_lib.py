@@ -199,6 +199,7 @@ class CwtPlan:
 
 
 _plans = OrderedDict()
+PLAN_CACHE = int(os.environ.get('SSQK_PLAN_CACHE', '64'))   # plans kept (LRU)
 
 
 def cwt_plan(device, N, L, n1, pad, kind, params, scales64, dt):
@@ -209,7 +210,7 @@ def cwt_plan(device, N, L, n1, pad, kind, params, scales64, dt):
     if plan is None:
         plan = CwtPlan(device, N, L, n1, pad, kind, params, scales64, dt)
         _plans[key] = plan
-        while len(_plans) > 32:
+        while len(_plans) > PLAN_CACHE:
             _plans.popitem(last=False)
     else:
         _plans.move_to_end(key)

@@ -64,7 +64,7 @@ def ssq_cwt_grid(N, wavelet, scales='log-piecewise', nv=None, fs=None, t=None,
     if g is None:
         g = _ssq_cwt_grid(N, wavelet, scales, nv, fs, t, ssq_freqs, maprange,
                           padtype, flipud)
-        if len(_GRID_CACHE) > 64:
+        if len(_GRID_CACHE) > 1024:
             _GRID_CACHE.clear()
         _GRID_CACHE[key] = g
     return g
