@@ -1,0 +1,90 @@
+"""N = 2^20 (the large-N configuration, two signals): properties that do
+not need a full float64 oracle at this size -- batch rows equal single
+signals, Wx of sampled scales against the oracle, conservation of the
+squeezed mass per column, and the adjoint's dot-product identity
+<A x, y> = <x, A^H y> for the cwt + squeeze map with the forward's rows."""
+import numpy as np
+import pytest
+
+from oracle import ssq_oracle as O
+from test_gpu_cwt import _psih, _wav
+
+pytestmark = pytest.mark.gpu
+N = 1 << 20
+
+
+def _signal(seed):
+    n = np.arange(N, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    t = n / N
+    x = (np.cos(2 * np.pi * 200 * np.log1p(9 * t) / np.log(10))     # hyperbolic-ish chirp
+         + 0.6 * np.cos(2 * np.pi * 0.17 * n) + 0.4 * np.cos(2 * np.pi * 0.31 * n)
+         + 0.2 * rng.standard_normal(N))
+    return x.astype(np.float32)
+
+
+def test_large_n_forward():
+    import torch
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+    wav = _wav('gmw')
+    xs = np.stack([_signal(0), _signal(1)])
+    print("large N forward", flush=True)
+    Tx, Wx, f, s = sq.ssq_cwt(torch.as_tensor(xs, device='cuda'), wav)
+    Tx1, Wx1, *_ = sq.ssq_cwt(torch.as_tensor(xs[1], device='cuda'), wav)
+    assert torch.equal(Wx[1], Wx1) and torch.equal(Tx[1], Tx1)
+    g = ssq_cwt_grid(N, wav)
+    na = g['na']
+    from ssqueezepy_amd import _lib
+    print(f"  na={na} plan: {list(_lib._plans.values())[-1].describe()}", flush=True)
+    pick = [0, na // 3, na // 2, (2 * na) // 3, na - 1]
+    Wo, _ = O.cwt(xs[0].astype(np.float64), _psih('gmw'), g['scales64'][pick])
+    Wg = Wx[0, pick].cpu().numpy()
+    for r, i in enumerate(pick):
+        e = np.linalg.norm(Wg[r] - Wo[r]) / np.linalg.norm(Wo[r])
+        print(f"  scale {i}: Wx rel-L2 {e:.2e}", flush=True)
+        assert e < 1e-5
+    # each column's squeezed mass: sum_k Tx[k, j] = sum_i c_i Wx[i, j] over binned i
+    from ssqueezepy_amd._cwt import run_cwt
+    bins = torch.empty(Wx.shape[1:], dtype=torch.int16, device='cuda')
+    Wtmp = torch.empty_like(Wx[0])
+    run_cwt(torch.as_tensor(xs[0], device='cuda'), g['wavelet'], g['scales64'], g['dt'],
+            'reflect', Wtmp, bins=bins, bp=g['bp'], gamma=10 * np.finfo(np.float32).eps)
+    c = torch.as_tensor(np.asarray(g['const'], dtype=np.float64), device='cuda').reshape(-1, 1)
+    valid = (bins.to(torch.int32) & 0xFFFF) != 0xFFFF
+    lhs = Tx[0].to(torch.complex128).sum(0)
+    rhs = (Wx[0].to(torch.complex128) * c * valid).sum(0)
+    e = (torch.linalg.norm(lhs - rhs) / torch.linalg.norm(rhs)).item()
+    print(f"  column mass conservation rel-L2 {e:.2e}", flush=True)
+    assert e < 1e-6
+
+
+def test_large_n_adjoint_dot_product():
+    import torch
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd import _lib
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid, _device_const
+    from ssqueezepy_amd._cwt import run_cwt
+    wav = _wav('gmw')
+    g = ssq_cwt_grid(N, wav)
+    na = g['na']
+    gen = torch.Generator(device='cuda').manual_seed(0)
+    x = torch.randn(2, N, device='cuda', generator=gen)
+    print("large N adjoint dot product", flush=True)
+    Wx = torch.empty((2, na, N), dtype=torch.complex64, device='cuda')
+    bins = torch.empty((2, na, N), dtype=torch.int16, device='cuda')
+    plan = run_cwt(x, g['wavelet'], g['scales64'], g['dt'], 'reflect', Wx, bins=bins,
+                   bp=g['bp'], gamma=10 * np.finfo(np.float32).eps)
+    cst = _device_const(g, 'cuda')
+    Tx = torch.empty_like(Wx)
+    _lib.squeeze(Wx, bins, cst, na, Tx)
+    yW = torch.randn(Wx.shape, dtype=torch.complex64, device='cuda', generator=gen)
+    yT = torch.randn(Tx.shape, dtype=torch.complex64, device='cuda', generator=gen)
+    lhs = ((Wx.to(torch.complex128).conj() * yW).real.sum()
+           + (Tx.to(torch.complex128).conj() * yT).real.sum()).item()
+    gx = torch.empty_like(x)
+    plan.adjoint(gx, gTx=yT, gWx=yW, bins=bins, cst=cst, nf=na)
+    rhs = (x.double() * gx.double()).sum().item()
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"  <Ax, y> {lhs:.9e}  <x, A^H y> {rhs:.9e}  rel {rel:.2e}", flush=True)
+    assert rel < 1e-4
