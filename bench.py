@@ -133,6 +133,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--N', type=int, default=160000)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--autotune', action='store_true',
+                    help='also run the plan autotuner and time the tuned plan')
     ap.add_argument('--no-batched', action='store_true',
                     help='skip the uneven-length batch comparison')
     ap.add_argument('--no-backward', action='store_true',
@@ -294,6 +296,20 @@ def main():
         res["fwd_bwd"] = fb
     if not args.no_batched:
         res["batched_uneven"] = batched_uneven(torch, sq)
+    if args.autotune:
+        tr = sq.autotune(N, wav)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms_t = (time.perf_counter() - t0) * 1e3 / args.steps
+        res["autotuned"] = {"opts": tr["best"], "ms_per_step": round(ms_t, 4),
+                            "value": round(units / (ms_t * 1e-3) / 1e9, 3),
+                            "tuning_seconds": round(tr["seconds"], 1),
+                            "candidates": [[o, round(m, 4)] for o, m in tr["results"]]}
     if not args.no_cpu:
         cb = cpu_baseline(x, g)
         res["cpu_baseline"] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind',

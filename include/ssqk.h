@@ -77,6 +77,23 @@ int ssqk_cwt_plan_create(ssqk_cwt_plan* plan, int64_t N, int64_t L,
                          int64_t n1, int pad_type, int wavelet_kind,
                          const double* wavelet_params, const double* scales,
                          int64_t na, double dt, double band_rel);
+/* Plan with tuning options (no reference counterpart: the reference has
+ * no plan; these are what ssqueezepy_amd.autotune searches):
+ *   side_streams  streams beside the caller's for the scale classes, 0..3
+ *                 (-1: SSQK_SIDE_STREAMS or 3);
+ *   os_group      scales per overlap-save workgroup, 1..8 (0: 8);
+ *   os_bias       factor on the overlap-save cost in the path choice
+ *                 (0: 1.0; below 1 moves scales to overlap-save). */
+typedef struct ssqk_plan_opts {
+  int32_t side_streams;
+  int32_t os_group;
+  double os_bias;
+} ssqk_plan_opts;
+int ssqk_cwt_plan_create2(ssqk_cwt_plan* plan, int64_t N, int64_t L,
+                          int64_t n1, int pad_type, int wavelet_kind,
+                          const double* wavelet_params, const double* scales,
+                          int64_t na, double dt, double band_rel,
+                          const ssqk_plan_opts* opts);
 int ssqk_cwt_plan_destroy(ssqk_cwt_plan plan);
 /* Number of scales the plan computes on each path: counts[0] band-limited
  * inverse FFTs, counts[1] overlap-save convolution, counts[2] full-length

@@ -13,6 +13,7 @@ from ._common import p2up, padsignal_np as padsignal
 from ._cwt import cwt
 from ._ssq_cwt import ssq_cwt, phase_cwt
 from ._batch import ssq_cwt_batch
+from ._autotune import autotune
 from ._stft import stft, get_window
 from ._ssq_stft import ssq_stft, phase_stft
 from .ssqueezing import ssqueeze

@@ -4,6 +4,8 @@ The host side keeps the reference's argument handling, padding parameters and
 scale grid; the transform itself (pad, FFT, psih generation, band-limited
 inverse FFTs and unpadding) is one call into ``ssqk_cwt_exec``.
 """
+import os
+
 import numpy as np
 
 from . import _lib
@@ -48,11 +50,31 @@ def _padding(N, padtype):
     return up, n1, padtype
 
 
+def autotune_env():
+    return os.environ.get('SSQK_AUTOTUNE', '0') == '1'
+
+
+def _maybe_autotune(device, N, L, n1, pad, wavelet, scales64, dt, padtype):
+    """SSQK_AUTOTUNE=1: tune the plan of this grid once (default scale grid
+    of ssq_cwt only; other grids keep the cost model's plan)."""
+    from . import _autotune
+    key = _lib.plan_key(device, N, L, n1, pad, wavelet.kind, _wavelet_kernel_params(wavelet),
+                        scales64, dt)
+    if _autotune.tuned_key_done(key) or key in _lib._tuned:
+        return
+    from ._ssq_cwt import ssq_cwt_grid
+    g = ssq_cwt_grid(N, wavelet, padtype=padtype)
+    if np.array_equal(g['scales64'], scales64) and g['dt'] == dt:
+        _autotune.autotune(N, wavelet, padtype=padtype)
+
+
 def run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=None, w=None,
             bins=None, bp=None, gamma=0., strict=True):
     """Launch the CWT plan for device signal(s) ``x_dev`` (float32 [B,] N)."""
     N = x_dev.shape[-1]
     L, n1, pad = _padding(N, padtype)
+    if autotune_env():
+        _maybe_autotune(x_dev.device, N, L, n1, pad, wavelet, scales64, dt, padtype)
     plan = _lib.cwt_plan(x_dev.device, N, L, n1, pad, wavelet.kind,
                          _wavelet_kernel_params(wavelet), scales64, dt)
     plan.exec(x_dev, Wx, dWx=dWx, w=w, bins=bins, bp=bp, gamma=gamma,

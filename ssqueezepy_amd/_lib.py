@@ -30,6 +30,16 @@ class BinParams(ctypes.Structure):
                 ('v1', c_double), ('d1', c_double)]
 
 
+class PlanOpts(ctypes.Structure):
+    """ssqk_plan_opts: side_streams (-1 default), os_group (0 default),
+    os_bias (0 default)."""
+    _fields_ = [('side_streams', c_int32), ('os_group', c_int32), ('os_bias', c_double)]
+
+
+def plan_opts(side_streams=-1, os_group=0, os_bias=0.):
+    return PlanOpts(int(side_streams), int(os_group), float(os_bias))
+
+
 _SIGS = {
     'ssqk_last_error': (ctypes.c_char_p, []),
     'ssqk_version': (c_int, []),
@@ -38,6 +48,10 @@ _SIGS = {
     'ssqk_cwt_plan_create': (c_int, [ctypes.POINTER(c_void_p), c_int64, c_int64,
                                      c_int64, c_int, c_int, c_double_p, c_double_p,
                                      c_int64, c_double, c_double]),
+    'ssqk_cwt_plan_create2': (c_int, [ctypes.POINTER(c_void_p), c_int64, c_int64,
+                                      c_int64, c_int, c_int, c_double_p, c_double_p,
+                                      c_int64, c_double, c_double,
+                                      ctypes.POINTER(PlanOpts)]),
     'ssqk_cwt_plan_destroy': (c_int, [c_void_p]),
     'ssqk_cwt_plan_paths': (c_int, [c_void_p, ctypes.POINTER(c_int64)]),
     'ssqk_cwt_plan_describe': (c_int, [c_void_p, ctypes.c_char_p, c_size_t]),
@@ -130,19 +144,20 @@ class CwtPlan:
     """Owner of an ssqk_cwt_plan (per-scale bands and task tables)."""
 
     def __init__(self, device, N, L, n1, pad, kind, params, scales64, dt,
-                 band_rel=1e-10):
+                 band_rel=1e-10, opts=None):
         import torch
         lib = load()
         self.device = device
         self.N, self.L, self.n1, self.na = int(N), int(L), int(n1), len(scales64)
+        self.opts = opts
         handle = c_void_p()
         p = (c_double * 2)(*params)
         s = np.ascontiguousarray(scales64, dtype=np.float64)
         with torch.cuda.device(device):
-            check(lib.ssqk_cwt_plan_create(
+            check(lib.ssqk_cwt_plan_create2(
                 ctypes.byref(handle), self.N, self.L, self.n1, PAD_CODES[pad],
                 kind, p, s.ctypes.data_as(c_double_p), self.na, float(dt),
-                float(band_rel)))
+                float(band_rel), ctypes.byref(opts) if opts is not None else None))
         self._h = handle
 
     def paths(self):
@@ -202,13 +217,31 @@ _plans = OrderedDict()
 PLAN_CACHE = int(os.environ.get('SSQK_PLAN_CACHE', '64'))   # plans kept (LRU)
 
 
+_tuned = {}          # plan key -> PlanOpts chosen by ssqueezepy_amd.autotune
+
+
+def plan_key(device, N, L, n1, pad, kind, params, scales64, dt):
+    return (str(device), int(N), int(L), int(n1), pad, kind, tuple(params),
+            np.asarray(scales64, dtype=np.float64).tobytes(), float(dt))
+
+
+def set_tuned(key, opts, plan=None):
+    """Use ``opts`` for plans of ``key`` from now on (and ``plan`` if given)."""
+    _tuned[key] = opts
+    if plan is not None:
+        _plans[key] = plan
+    else:
+        _plans.pop(key, None)
+
+
 def cwt_plan(device, N, L, n1, pad, kind, params, scales64, dt):
-    """Cached CwtPlan for (device, sizes, wavelet, scales, dt)."""
-    key = (str(device), N, L, n1, pad, kind, tuple(params),
-           np.asarray(scales64, dtype=np.float64).tobytes(), float(dt))
+    """Cached CwtPlan for (device, sizes, wavelet, scales, dt), built with the
+    autotuned options of that key if ``autotune`` chose some."""
+    key = plan_key(device, N, L, n1, pad, kind, params, scales64, dt)
     plan = _plans.get(key)
     if plan is None:
-        plan = CwtPlan(device, N, L, n1, pad, kind, params, scales64, dt)
+        plan = CwtPlan(device, N, L, n1, pad, kind, params, scales64, dt,
+                       opts=_tuned.get(key))
         _plans[key] = plan
         while len(_plans) > PLAN_CACHE:
             _plans.popitem(last=False)

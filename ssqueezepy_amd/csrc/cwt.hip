@@ -947,6 +947,9 @@ hipError_t upload(T** d, const std::vector<T>& h) {
 
 struct PlanOpts {
   bool allow_os = true;
+  int os_group = 8;           // scales per overlap-save group (1..8)
+  double os_bias = 1.0;       // factor on the overlap-save cost estimate
+  int side_streams = -1;      // -1: SSQK_SIDE_STREAMS or 3
 };
 
 int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, int pad_type,
@@ -1063,8 +1066,8 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       const int H = (int)std::ceil(T1 * s * 1.02) + 2;
       for (int F = 256; F <= 4096; F *= 2) {
         if (F < 2 * H + 64 || 2 * (int64_t)H + 1 > N) continue;
-        const double c = (std::log2((double)F) + 3.0) * F / (double)(F - 2 * H) +
-                         std::log2((double)F) / 8.0;
+        const double c = ((std::log2((double)F) + 3.0) * F / (double)(F - 2 * H) +
+                          std::log2((double)F) / 8.0) * opt.os_bias;
         if (c < cost) { cost = c; bestF = F; bestH = H; }
       }
     }
@@ -1268,7 +1271,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       g.ns = 0;
       g.H = 0;
       g.rank = (int)os_scales.size();
-      while (i < na && os_F[i] == F && g.ns < 8) {
+      while (i < na && os_F[i] == F && g.ns < std::max(1, std::min(8, opt.os_group))) {
         g.H = std::max(g.H, os_H[i]);
         // F-grid band of psih(s xi) above band_rel, two bins of margin
         const double d = 2 * M_PI / (double)F * scales[i];
@@ -1344,6 +1347,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
   }
   if (const char* ev = std::getenv("SSQK_SIDE_STREAMS"))
     p->nside = std::max(0, std::min(ssqk_cwt_plan_s::kSide, std::atoi(ev)));
+  if (opt.side_streams >= 0) p->nside = std::min(ssqk_cwt_plan_s::kSide, opt.side_streams);
   for (int k = 0; k < ssqk_cwt_plan_s::kSide && e == hipSuccess; ++k) {
     e = hipStreamCreateWithFlags(&p->side[k], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_done[k], hipEventDisableTiming);
@@ -1367,10 +1371,11 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
 
 }  // namespace
 
-extern "C" int ssqk_cwt_plan_create(ssqk_cwt_plan* out, int64_t N, int64_t L, int64_t n1,
-                                    int pad_type, int wavelet_kind,
-                                    const double* wavelet_params, const double* scales,
-                                    int64_t na, double dt, double band_rel) {
+extern "C" int ssqk_cwt_plan_create2(ssqk_cwt_plan* out, int64_t N, int64_t L, int64_t n1,
+                                     int pad_type, int wavelet_kind,
+                                     const double* wavelet_params, const double* scales,
+                                     int64_t na, double dt, double band_rel,
+                                     const ssqk_plan_opts* opts) {
   if (!out || !wavelet_params || !scales) {
     ssqk_host::set_error("null argument");
     return SSQK_ERR_ARG;
@@ -1414,11 +1419,29 @@ extern "C" int ssqk_cwt_plan_create(ssqk_cwt_plan* out, int64_t N, int64_t L, in
   const char* env = std::getenv("SSQK_DISABLE_OVERLAP_SAVE");
   PlanOpts opt;
   opt.allow_os = !(env && env[0] == '1');
+  if (opts) {
+    if (opts->os_group < 0 || opts->os_group > 8 || opts->side_streams > 3 ||
+        opts->os_bias < 0) {
+      ssqk_host::set_error("bad plan options");
+      return SSQK_ERR_ARG;
+    }
+    if (opts->os_group) opt.os_group = opts->os_group;
+    if (opts->os_bias > 0) opt.os_bias = opts->os_bias;
+    opt.side_streams = opts->side_streams;
+  }
   const int rc = plan_create_impl(&p, N, L, n1, pad_type, wav, center, scales, na, dt,
                                   band_rel, opt);
   if (rc) return rc;
   *out = p;
   return SSQK_OK;
+}
+
+extern "C" int ssqk_cwt_plan_create(ssqk_cwt_plan* out, int64_t N, int64_t L, int64_t n1,
+                                    int pad_type, int wavelet_kind,
+                                    const double* wavelet_params, const double* scales,
+                                    int64_t na, double dt, double band_rel) {
+  return ssqk_cwt_plan_create2(out, N, L, n1, pad_type, wavelet_kind, wavelet_params, scales,
+                               na, dt, band_rel, nullptr);
 }
 
 extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {

@@ -45,3 +45,25 @@ def test_batch_copy_outputs_survive_next_pass():
     a0 = a[0][0].clone()
     sq.ssq_cwt_batch([chirp_tone_noise(n, 8) for n in lengths], wav, copy=True)
     assert torch.equal(a[0][0], a0)
+
+
+def test_autotune_registers_a_plan_and_keeps_parity():
+    import torch
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd import _lib
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+    from oracle import ssq_oracle as O
+    from test_gpu_cwt import _psih, _wav
+    N = 7000
+    wav = _wav('gmw')
+    res = sq.autotune(N, wav, space=dict(side_streams=(0, 3), os_group=(8,),
+                                         os_bias=(0.8, 1.25)), reps=2)
+    print("autotune", res['best'], round(res['ms'], 4), flush=True)
+    assert len(res['results']) == 4 and res['best'] in [r[0] for r in res['results']]
+    assert len(_lib._tuned) >= 1
+    x = chirp_tone_noise(N, 2)
+    Wx, _ = sq.cwt(torch.as_tensor(x, device='cuda'), wav)
+    g = ssq_cwt_grid(N, wav)
+    Wo, _ = O.cwt(x.astype(np.float64), _psih('gmw'), g['scales64'])
+    e = np.linalg.norm(Wx.cpu().numpy() - Wo) / np.linalg.norm(Wo)
+    assert e < 1e-5
