@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -171,11 +172,12 @@ __global__ void k_bins_w(const float* __restrict__ w, int64_t total, BinDev bp,
 // ascending scale order by the lowest such lane, the others target their
 // dummy row.  Quad 0 then quad 1 update LDS, so every accumulator receives
 // its column's scales in ascending order: deterministic, no atomics, Tx
-// written once.  kSqUL groups are loaded ahead of the ones being summed.
+// written once.  Two blocks of kSqUL groups are in flight alternately.
 constexpr int kSqQPC = 2;                    // quads per column
 constexpr int kSqGS = 4 * kSqQPC;            // scales per group
 constexpr int kSqUL = 4;                     // groups per prefetch block
 constexpr int kSqCPW = 16 / kSqQPC;          // columns per wave
+constexpr bool kSqXcd = true;                // XCD-aware tile order
 
 template <int S>
 __device__ __forceinline__ int quad_bcast(int v) {
@@ -195,7 +197,17 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
   constexpr int RS = W + 1;
   extern __shared__ float2 s_acc[];          // [R + 4][RS]
   const int b = blockIdx.y;
-  const int64_t j0 = (int64_t)blockIdx.x * W;
+  // XCD-aware tile order: block x goes to XCD x % 8, which takes a
+  // contiguous share of the column tiles, so the bins line two neighbouring
+  // tiles share is fetched into one L2 once
+  int tile = blockIdx.x;
+  if (kSqXcd) {
+    const int ntiles = (int)((N + W - 1) / W);
+    const int x = tile & 7, jx = tile >> 3, qx = ntiles >> 3, rx = ntiles & 7;
+    if (jx >= qx + (x < rx ? 1 : 0)) return;
+    tile = x * qx + (x < rx ? x : rx) + jx;
+  }
+  const int64_t j0 = (int64_t)tile * W;
   const int tid = threadIdx.x;
   for (int e = tid; e < (R + 4) * RS; e += NT) s_acc[e] = make_float2(0.f, 0.f);
   __syncthreads();
@@ -224,21 +236,14 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
       cc[u] = cst[ib + (in ? sl : 0)];
     }
   };
-  float2 wA[kSqUL];
-  int kA[kSqUL];
-  float cA[kSqUL];
-  load(0, wA, kA, cA);
-  for (int i0 = 0; i0 < na; i0 += kSqGS * kSqUL) {
-    float2 wB[kSqUL];
-    int kB[kSqUL];
-    float cB[kSqUL];
-    load(i0 + kSqGS * kSqUL, wB, kB, cB);
+  // sum the kSqUL groups of one prefetch block in ascending scale order
+  auto consume = [&](int i0, const float2* w, const int* kk, const float* cc) {
 #pragma unroll
     for (int u = 0; u < kSqUL; ++u) {
       const int ib = i0 + kSqGS * u;
-      const int kr = (live && ib + sl < na) ? kA[u] - r0 : -1;
+      const int kr = (live && ib + sl < na) ? kk[u] - r0 : -1;
       const int k = (kr >= 0 && kr < R) ? kr : -1;
-      const float pre = wA[u].x * cA[u], pim = wA[u].y * cA[u];
+      const float pre = w[u].x * cc[u], pim = w[u].y * cc[u];
       const int k0 = quad_bcast<0>(k), k1 = quad_bcast<1>(k);
       const int k2 = quad_bcast<2>(k), k3 = quad_bcast<3>(k);
       const float x0 = quad_bcast<0>(pre), x1 = quad_bcast<1>(pre);
@@ -254,26 +259,48 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
       const int row = (dup || k < 0) ? dummy : k;
 #pragma unroll
       for (int pp = 0; pp < kSqQPC; ++pp) {
-        float2* a = s_acc + (p == pp ? row : dummy) * RS + c;
-        float2 acc = *a;
+        float2* acc_p = s_acc + (p == pp ? row : dummy) * RS + c;
+        float2 acc = *acc_p;
         acc.x += sx;
         acc.y += sy;
-        *a = acc;
+        *acc_p = acc;
       }
     }
-#pragma unroll
-    for (int u = 0; u < kSqUL; ++u) {
-      kA[u] = kB[u];
-      wA[u] = wB[u];
-      cA[u] = cB[u];
-    }
+  };
+  // two prefetch blocks in flight alternately (ping-pong, no register
+  // copies that would wait for the newer loads)
+  constexpr int BS = kSqGS * kSqUL;          // scales per prefetch block
+  float2 wA[kSqUL], wB[kSqUL];
+  int kA[kSqUL], kB[kSqUL];
+  float cA[kSqUL], cB[kSqUL];
+  // (the empty asm statements keep the compiler from reordering the loads
+  // of one block past the other's, which would make every wait drain both)
+  load(0, wA, kA, cA);
+  asm volatile("" ::: "memory");
+  load(BS, wB, kB, cB);
+  for (int i0 = 0; i0 < na; i0 += 2 * BS) {
+    asm volatile("" ::: "memory");
+    consume(i0, wA, kA, cA);
+    asm volatile("" ::: "memory");
+    load(i0 + 2 * BS, wA, kA, cA);
+    asm volatile("" ::: "memory");
+    consume(i0 + BS, wB, kB, cB);             // rows past na are masked
+    asm volatile("" ::: "memory");
+    load(i0 + 3 * BS, wB, kB, cB);
   }
   __syncthreads();
   float2* tp = Tx + (int64_t)b * out_stride + (int64_t)r0 * N;
   for (int e = tid; e < R * W; e += NT) {
     const int k = e / W, cc = e - k * W;
     const int64_t jj = j0 + cc;
-    if (jj < N) tp[(int64_t)k * N + jj] = s_acc[k * RS + cc];
+    // streaming stores: Tx bypasses the caches, which keep Wx and bins
+    if (jj < N) {
+      const float2 v = s_acc[k * RS + cc];
+      const unsigned long long bits = (unsigned long long)__float_as_uint(v.x) |
+                                      ((unsigned long long)__float_as_uint(v.y) << 32);
+      __builtin_nontemporal_store(bits,
+                                  reinterpret_cast<unsigned long long*>(tp + (int64_t)k * N + jj));
+    }
   }
 }
 
@@ -368,16 +395,17 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
     ssqk_host::set_error("bad arguments to ssqk_squeeze");
     return SSQK_ERR_ARG;
   }
-  // accumulators [R + 4][W + 1] complex in 160 KiB of LDS: 32 columns while
-  // all rows fit, else 16 or 8; taller outputs are done in row windows
+  // accumulators [R + 4][W + 1] complex in 160 KiB of LDS: 64 columns while
+  // all rows fit (nf <= 311), else 32, 16 or 8; taller outputs are done in
+  // row windows
   constexpr size_t kLds = 160 * 1024;
   auto fits = [&](int64_t rows, int w) {
     return (size_t)(rows + 4) * (w + 1) * sizeof(float2) <= kLds;
   };
-  int W = 32;
+  int W = 64;
   while (W > 8 && !fits(nf, W)) W /= 2;
   const int64_t R = fits(nf, W) ? nf : (int64_t)(kLds / ((W + 1) * sizeof(float2))) - 4;
-  static bool attr_set[3] = {false, false, false};
+  static bool attr_set[4] = {};
   auto launch = [&](auto kern, int wi) -> int {
     if (!attr_set[wi]) {
       SSQK_CHECK(hipFuncSetAttribute((const void*)kern,
@@ -387,7 +415,8 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
     for (int64_t r0 = 0; r0 < nf; r0 += R) {
       const int64_t rows = std::min<int64_t>(R, nf - r0);
       const size_t lds = (size_t)(rows + 4) * (W + 1) * sizeof(float2);
-      dim3 grid((unsigned)((N + W - 1) / W), (unsigned)batch);
+      const unsigned ntiles = (unsigned)((N + W - 1) / W);
+      dim3 grid(kSqXcd ? (ntiles + 7) / 8 * 8 : ntiles, (unsigned)batch);
       hipLaunchKernelGGL(kern, grid, dim3(W * 4 * kSqQPC), lds, (hipStream_t)stream,
                          (const float2*)Wx, bins, cst, (int)na, (int)r0, (int)rows, N,
                          na * N, nf * N, (float2*)Tx);
@@ -395,7 +424,8 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
     }
     return SSQK_OK;
   };
-  if (W == 32) return launch(k_squeeze<32>, 0);
-  if (W == 16) return launch(k_squeeze<16>, 1);
-  return launch(k_squeeze<8>, 2);
+  if (W == 64) return launch(k_squeeze<64>, 0);
+  if (W == 32) return launch(k_squeeze<32>, 1);
+  if (W == 16) return launch(k_squeeze<16>, 2);
+  return launch(k_squeeze<8>, 3);
 }

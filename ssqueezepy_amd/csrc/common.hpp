@@ -55,40 +55,29 @@ __device__ __forceinline__ int bin_exact(double w, const BinDev& p) {
   return p.flip ? p.omax - ki : ki;
 }
 
-// Bin of a phase-transform value given as the float32 inputs A+iB (dWx) and
-// C+iD (Wx).  The fast float32 evaluation is used unless the value lies
-// within 1e-3 of a rounding edge (or of the piecewise transition), where the
-// float64 evaluation of the same inputs decides.
-__device__ __forceinline__ int bin_from_dwx(float A, float B, float C, float D,
-                                            const BinDev& p) {
-  const float m2 = C * C + D * D;
-  const float w = fabsf((B * C - A * D) / (m2 * kTwoPiF));
+// Bin of a float32 phase-transform value w by the float32 evaluation, or -1
+// when w lies within 1e-3 bins of a rounding edge (or near the piecewise
+// transition, or out of range): the float64 evaluation then decides.
+__device__ __forceinline__ int bin_fast_w(float w, const BinDev& p) {
   float u;
   bool near = false;
+  const float wl = p.type == 0 ? 0.0f : __log2f(w);
   if (p.type == 0) {
     u = (w - p.v0f) * p.inv_d0f;
+  } else if (p.type == 2) {
+    const float dd = wl - p.v1f;
+    near = fabsf(dd) < 1e-4f * (1.0f + fabsf(p.v1f));
+    u = dd > 0.0f ? dd * p.inv_d1f : (wl - p.v0f) * p.inv_d0f;
   } else {
-    const float wl = __log2f(w);
-    if (p.type == 2) {
-      const float dd = wl - p.v1f;
-      near = fabsf(dd) < 1e-4f * (1.0f + fabsf(p.v1f));
-      u = dd > 0.0f ? dd * p.inv_d1f : (wl - p.v0f) * p.inv_d0f;
-    } else {
-      u = (wl - p.v0f) * p.inv_d0f;
-    }
+    u = (wl - p.v0f) * p.inv_d0f;
   }
   const float fr = u - floorf(u);
   near = near || fabsf(fr - 0.5f) < 1e-3f || !(fabsf(u) < 1e6f);
-  if (near) {
-    const double wd = fabs(((double)B * C - (double)A * D) /
-                           (((double)C * C + (double)D * D) * kTwoPi));
-    return bin_exact(wd, p);
-  }
+  if (near) return -1;
   float k;
   if (p.type == 0 || p.type == 1) {
     k = fminf(rintf(fmaxf(u, 0.0f)), (float)p.omax);
   } else {
-    const float wl = __log2f(w);
     if (wl > p.v1f) k = fminf(rintf(u) + (float)p.idx1, (float)p.omax);
     else k = fmaxf(rintf(u), 0.0f);
   }
@@ -96,36 +85,34 @@ __device__ __forceinline__ int bin_from_dwx(float A, float B, float C, float D,
   return p.flip ? p.omax - ki : ki;
 }
 
+// Phase transform |Im(dWx / Wx)| / 2pi of dWx = A+iB, Wx = C+iD in float32
+// with the hardware reciprocal (1 ulp; the bin margins cover its error).
+__device__ __forceinline__ float phase_fast(float A, float B, float C, float D) {
+  const float m2 = C * C + D * D;
+  return fabsf(B * C - A * D) * __builtin_amdgcn_rcpf(m2 * kTwoPiF);
+}
+
+// The float64 evaluation of the same inputs, for the points bin_fast_w
+// leaves open.
+__device__ __forceinline__ int bin_edge_dwx(float A, float B, float C, float D,
+                                            const BinDev& p) {
+  const double wd = fabs(((double)B * C - (double)A * D) /
+                         (((double)C * C + (double)D * D) * kTwoPi));
+  return bin_exact(wd, p);
+}
+
+// Bin of the phase transform of dWx = A+iB, Wx = C+iD.
+__device__ __forceinline__ int bin_from_dwx(float A, float B, float C, float D,
+                                            const BinDev& p) {
+  const int k = bin_fast_w(phase_fast(A, B, C, D), p);
+  return k >= 0 ? k : bin_edge_dwx(A, B, C, D, p);
+}
+
 // Same for an already computed float32 w (indexed_sum_onfly path,
 // algos.py:153-250): caller has checked that w is finite.
 __device__ __forceinline__ int bin_from_w(float w, const BinDev& p) {
-  float u;
-  bool near = false;
-  if (p.type == 0) {
-    u = (w - p.v0f) * p.inv_d0f;
-  } else {
-    const float wl = __log2f(w);
-    if (p.type == 2) {
-      const float dd = wl - p.v1f;
-      near = fabsf(dd) < 1e-4f * (1.0f + fabsf(p.v1f));
-      u = dd > 0.0f ? dd * p.inv_d1f : (wl - p.v0f) * p.inv_d0f;
-    } else {
-      u = (wl - p.v0f) * p.inv_d0f;
-    }
-  }
-  const float fr = u - floorf(u);
-  near = near || fabsf(fr - 0.5f) < 1e-3f || !(fabsf(u) < 1e6f);
-  if (near) return bin_exact((double)w, p);
-  float k;
-  if (p.type == 0 || p.type == 1) {
-    k = fminf(rintf(fmaxf(u, 0.0f)), (float)p.omax);
-  } else {
-    const float wl = __log2f(w);
-    if (wl > p.v1f) k = fminf(rintf(u) + (float)p.idx1, (float)p.omax);
-    else k = fmaxf(rintf(u), 0.0f);
-  }
-  const int ki = (int)k;
-  return p.flip ? p.omax - ki : ki;
+  const int k = bin_fast_w(w, p);
+  return k >= 0 ? k : bin_exact((double)w, p);
 }
 
 inline BinDev make_bin_dev(const ssqk_bin_params* bp) {

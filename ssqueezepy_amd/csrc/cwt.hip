@@ -111,11 +111,14 @@ __device__ __forceinline__ int64_t pad_src(int64_t n, int64_t N, int64_t n1, int
   }
 }
 
-// Shared output stage: z, zd are Wx and dWx at padded position n.
-__device__ __forceinline__ void emit(const CwtArgs& a, int b, int i, int64_t n,
-                                     float2 z, float2 zd) {
+// Shared output stage: z, zd are Wx and dWx at padded position n.  Returns
+// false when the point's bin lies at a float32 rounding edge: the caller
+// then finishes the point with emit_edge (the float64 evaluation, long code
+// kept out of unrolled loops).
+__device__ __forceinline__ bool emit_fast(const CwtArgs& a, int b, int i, int64_t n,
+                                          float2 z, float2 zd) {
   const int64_t j = n - a.n1;
-  if (j < 0 || j >= a.N) return;
+  if (j < 0 || j >= a.N) return true;
   const int64_t o = (int64_t)b * a.out_stride + (int64_t)i * a.N + j;
   a.Wx[o] = z;
   if (a.dWx) a.dWx[o] = zd;
@@ -127,14 +130,34 @@ __device__ __forceinline__ void emit(const CwtArgs& a, int b, int i, int64_t n,
     if (a.w) a.w[o] = wv;
   }
   if (a.bins) {
-    uint16_t k = (uint16_t)kNoBin;
+    int k = kNoBin;
     if (a.strict) {
-      if (m2 > a.gamma2) k = (uint16_t)bin_from_dwx(zd.x, zd.y, z.x, z.y, a.bp);
+      if (m2 > a.gamma2) k = bin_fast_w(phase_fast(zd.x, zd.y, z.x, z.y), a.bp);
     } else if (!isinf(wv)) {
-      k = (uint16_t)bin_from_w(wv, a.bp);
+      k = bin_fast_w(wv, a.bp);
     }
-    a.bins[o] = k;
+    if (k < 0) return false;
+    a.bins[o] = (uint16_t)k;
   }
+  return true;
+}
+
+__device__ __forceinline__ void emit_edge(const CwtArgs& a, int b, int i, int64_t n,
+                                          float2 z, float2 zd) {
+  const int64_t o = (int64_t)b * a.out_stride + (int64_t)i * a.N + (n - a.n1);
+  int k;
+  if (a.strict) {
+    k = bin_edge_dwx(zd.x, zd.y, z.x, z.y, a.bp);
+  } else {
+    const float m2 = z.x * z.x + z.y * z.y;
+    k = bin_exact((double)fabsf((zd.y * z.x - zd.x * z.y) / (m2 * kTwoPiF)), a.bp);
+  }
+  a.bins[o] = (uint16_t)k;
+}
+
+__device__ __forceinline__ void emit(const CwtArgs& a, int b, int i, int64_t n,
+                                     float2 z, float2 zd) {
+  if (!emit_fast(a, b, i, n, z, zd)) emit_edge(a, b, i, n, z, zd);
 }
 
 // X = psih(s xi_k) v and Xd = X * i xi_k / dt for a positive bin k, where
@@ -177,6 +200,12 @@ __global__ __launch_bounds__(256) void k_psih_tab(WavDev w, const double* __rest
 
 // ------------------------------------------------------------- band path
 // task = {scale i, first phase a0, band start ka, log2 D}
+// With L = M D, band bins k = ka + m (m < M) and outputs n = a + D q
+// (phase a < D, q < M):
+//   W[a + D q] = (1/L) sum_m Y[m] e(k a / L) e(((ka + m) mod M) q / M),
+// i.e. the M-point inverse DFT of Z[(ka + m) mod M] = Y[m] e(k a / L).
+// Placing bin m at the cyclic position (ka + m) mod M absorbs the output
+// twiddle e(ka q / M), so the FFT's outputs are Wx and dWx as they stand.
 template <int M>
 __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const int b,
                                           const float2* __restrict__ tw,
@@ -187,8 +216,10 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
   const int L = a.L, half = L >> 1;
   const float2* xh = a.xh + (int64_t)b * a.xh_stride;
   const float* ps = a.psih + a.psih_off[i];
+  const float invL = 1.0f / (float)L;   // a power of two: scaling is exact
 
-  // band samples into row 0 (the table is zero past the Nyquist bin)
+  // band samples into row 0 at their cyclic positions (the table is zero
+  // past the Nyquist bin)
   constexpr int IM = (M + kNT - 1) / kNT;
   {
     float2 v[IM];
@@ -204,30 +235,34 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
       const int m = threadIdx.x + it * kNT;
       if (m < M) {
         float2 X, Xd;
-        band_value(a, v[it], pv[it], ka + m, X, Xd);
-        sW[Lds<M>::idx(0, m)] = X;
-        sD[Lds<M>::idx(0, m)] = Xd;
+        band_value(a, v[it], pv[it] * invL, ka + m, X, Xd);
+        const int j = (ka + m) & (M - 1);
+        sW[Lds<M>::idx(0, j)] = X;
+        sD[Lds<M>::idx(0, j)] = Xd;
       }
     }
   }
   __syncthreads();
-  // phase ramps w_L^{m (a0 + r)}: rows r >= 1 first, then row 0 in place
+  // phase ramps e(k (a0 + r) / L) of position j (bin k = ka + ((j - ka) mod M)):
+  // rows r >= 1 first, then row 0 in place.  k (a0 + r) is reduced mod L, a
+  // divisor of 2^32, so 32-bit wrap-around is exact.
   constexpr int IR = (ROWS * M + kNT - 1) / kNT;
   {
     float2 ph[IR];
 #pragma unroll
     for (int it = 0; it < IR; ++it) {
       const int e = threadIdx.x + it * kNT;
-      const int r = e / M, m = e - r * M;
-      ph[it] = phase_tab((uint64_t)m * (uint64_t)(a0 + r), a.ptab, L, a.pS);
+      const int r = e / M, j = e - r * M;
+      const uint32_t k = (uint32_t)(ka + ((j - ka) & (M - 1)));
+      ph[it] = phase_tab(k * (uint32_t)(a0 + r), a.ptab, L, a.pS);
     }
 #pragma unroll
     for (int it = 0; it < IR; ++it) {
       const int e = threadIdx.x + it * kNT;
-      const int r = e / M, m = e - r * M;
+      const int r = e / M, j = e - r * M;
       if (r >= 1 && r < ROWS) {
-        sW[Lds<M>::idx(r, m)] = cmul(sW[Lds<M>::idx(0, m)], ph[it]);
-        sD[Lds<M>::idx(r, m)] = cmul(sD[Lds<M>::idx(0, m)], ph[it]);
+        sW[Lds<M>::idx(r, j)] = cmul(sW[Lds<M>::idx(0, j)], ph[it]);
+        sD[Lds<M>::idx(r, j)] = cmul(sD[Lds<M>::idx(0, j)], ph[it]);
       }
     }
     __syncthreads();
@@ -243,25 +278,25 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
   float2* const bufs[2] = {sW, sD};
   lds_fft<M, ROWS, +1, kNT, 2>(bufs, tw);
 
-  const float invL = 1.0f / (float)L;
-  float2 ph[IR];
-#pragma unroll
-  for (int it = 0; it < IR; ++it) {
-    const int e = threadIdx.x + it * kNT;
-    const int r = e % ROWS, q = e / ROWS;
-    const int64_t n = a0 + r + (int64_t)D * q;
-    ph[it] = phase_tab((uint64_t)ka * (uint64_t)n, a.ptab, L, a.pS);
-  }
+  static_assert(IR <= 32, "edge mask");
+  uint32_t edge = 0;
 #pragma unroll
   for (int it = 0; it < IR; ++it) {
     const int e = threadIdx.x + it * kNT;
     const int r = e % ROWS, q = e / ROWS;
     const int ph_a = a0 + r;
     const int64_t n = ph_a + (int64_t)D * q;
-    if (e < ROWS * M && ph_a < D && n >= a.n1 && n < (int64_t)a.n1 + a.N) {
-      const float2 p = make_float2(ph[it].x * invL, ph[it].y * invL);
-      emit(a, b, i, n, cmul(sW[Lds<M>::idx(r, q)], p), cmul(sD[Lds<M>::idx(r, q)], p));
-    }
+    if (e < ROWS * M && ph_a < D && n >= a.n1 && n < (int64_t)a.n1 + a.N &&
+        !emit_fast(a, b, i, n, sW[Lds<M>::idx(r, q)], sD[Lds<M>::idx(r, q)]))
+      edge |= 1u << it;
+  }
+  while (edge) {
+    const int it = __builtin_ctz(edge);
+    edge &= edge - 1;
+    const int e = threadIdx.x + it * kNT;
+    const int r = e % ROWS, q = e / ROWS;
+    emit_edge(a, b, i, a0 + r + (int64_t)D * q, sW[Lds<M>::idx(r, q)],
+              sD[Lds<M>::idx(r, q)]);
   }
 }
 
@@ -336,7 +371,7 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullA(CwtArgs a, const int4* __
   for (int it = 0; it < IA; ++it) {
     const int e = threadIdx.x + it * kNT;
     const int g = e % G, n2 = e / G;
-    ph[it] = phase_tab((uint64_t)(k1_0 + g) * (uint64_t)n2, a.ptab, a.L, a.pS);
+    ph[it] = phase_tab((uint32_t)(k1_0 + g) * (uint32_t)n2, a.ptab, a.L, a.pS);
   }
 #pragma unroll
   for (int it = 0; it < IA; ++it) {
@@ -574,7 +609,7 @@ __global__ __launch_bounds__(kNT) void k_fwd_A(FwdArgs f, const float2* __restri
   for (int e = threadIdx.x; e < G * L2; e += kNT) {
     const int g = e % G, n2 = e / G;
     const int k1 = k1_0 + g;
-    const float2 ph = cconj(phase_tab((uint64_t)k1 * (uint64_t)n2, f.ptab, f.L, f.pS));
+    const float2 ph = cconj(phase_tab((uint32_t)k1 * (uint32_t)n2, f.ptab, f.L, f.pS));
     scr[((int64_t)b * L2 + n2) * L1 + k1] = cmul(s[Lds<L2>::idx(g, n2)], ph);
   }
 }
@@ -1808,7 +1843,7 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
       const int r = e / M, q = e - r * M;
       const int64_t j = ab + r + D * q - a.n1;
       v[it] = yb[(j >= 0 && j < a.N) ? j : 0];
-      w[it] = cconj(phase_tab((uint64_t)ka * (uint64_t)q * (uint64_t)D, a.ptab, L, a.pS));
+      w[it] = cconj(phase_tab((uint32_t)ka * (uint32_t)q * (uint32_t)D, a.ptab, L, a.pS));
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -1824,7 +1859,7 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
     for (int it = 0; it < IT; ++it) {
       const int e = threadIdx.x + it * NT;
       const int r = e / M, m = e - r * M;
-      const float2 ph_ = cconj(phase_tab((uint64_t)(ka + m) * (uint64_t)(ab + r), a.ptab, L,
+      const float2 ph_ = cconj(phase_tab((uint32_t)(ka + m) * (uint32_t)(ab + r), a.ptab, L,
                                          a.pS));
       const float2 y = cmul(s[Lds<M>::idx(r, m)], ph_);
       acc[it].x += y.x;
@@ -2033,7 +2068,7 @@ __global__ __launch_bounds__(kAdjNT4) void k_adj_fullA(AdjArgs a, const int4* __
   const int64_t base = ((int64_t)slot * batch + b) * (int64_t)L2;
   for (int e = threadIdx.x; e < G * L2; e += kAdjNT4) {
     const int g = e % G, n2 = e / G;
-    const float2 ph = cconj(phase_tab((uint64_t)(k1_0 + g) * (uint64_t)n2, a.ptab, a.L, a.pS));
+    const float2 ph = cconj(phase_tab((uint32_t)(k1_0 + g) * (uint32_t)n2, a.ptab, a.L, a.pS));
     scr[(base + n2) * L1 + k1_0 + g] = cmul(s[Lds<L2>::idx(g, n2)], ph);
   }
 }
@@ -2128,7 +2163,7 @@ __global__ __launch_bounds__(kNT) void k_adj_ifftA(const float2* __restrict__ S,
   for (int e = threadIdx.x; e < G * L2; e += kNT) {
     const int g = e % G, n2 = e / G;
     const int k1 = k1_0 + g;
-    const float2 ph = phase_tab((uint64_t)k1 * (uint64_t)n2, ptab, L, pS);
+    const float2 ph = phase_tab((uint32_t)k1 * (uint32_t)n2, ptab, L, pS);
     scr[((int64_t)b * L2 + n2) * L1 + k1] = cmul(s[Lds<L2>::idx(g, n2)], ph);
   }
 }

@@ -287,10 +287,11 @@ __device__ __forceinline__ float2 unit_phase(uint64_t t, uint32_t L) {
 // exp(+2 pi i t / L) from a two-level table built by
 // ssqk_host::phase_table(L): tab[j] = e(j / L) for j < 2^S and
 // tab[2^S + h] = e(h 2^S / L), so e(t/L) = tab[2^S + (t >> S)] * tab[t & (2^S-1)]
-// (two L1-resident loads and one complex product, ~1.5 ulp).
-__device__ __forceinline__ float2 phase_tab(uint64_t t, const float2* __restrict__ tab,
+// (two L1-resident loads and one complex product, ~1.5 ulp).  t is taken
+// mod 2^32: L divides 2^32, so products that wrap around stay exact mod L.
+__device__ __forceinline__ float2 phase_tab(uint32_t t, const float2* __restrict__ tab,
                                             uint32_t L, int S) {
-  const uint32_t r = (uint32_t)(t & (uint64_t)(L - 1));
+  const uint32_t r = t & (L - 1);
   const float2 lo = tab[r & ((1u << S) - 1u)];
   const float2 hi = tab[(1u << S) + (r >> S)];
   return cmul(hi, lo);
