@@ -94,6 +94,47 @@ def batched_uneven(torch, sq, n_signals=256, passes=3):
             "unit": "Gscale*samples/s"}
 
 
+def fwd_bwd_batch(torch, sq, B=32, N=16384, steps=10):
+    """Training-style step on a batch of 32 signals N=16,384 (chirp + tone +
+    noise, seeds 0..31, GMW default): loss mean(||Tx| - |Tx_target||^2) with
+    the target from seeds 100..131, gradient w.r.t. x through the adjoint.
+    Forward alone, forward + backward, and their ratio."""
+    dev = torch.device('cuda', 0)
+    x = torch.as_tensor(np.stack([chirp_tone_noise(N, s) for s in range(B)]), device=dev)
+    xt = torch.as_tensor(np.stack([chirp_tone_noise(N, 100 + s) for s in range(B)]),
+                         device=dev)
+    tgt = sq.ssq_cwt(xt)[0].abs()
+
+    def fwd():
+        return sq.ssq_cwt(x)[0]
+
+    def fwd_bwd():
+        xg = x.detach().requires_grad_(True)
+        T = sq.ssq_cwt(xg)[0]
+        ((T.abs() - tgt) ** 2).mean().backward()
+        return xg.grad
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    ms_f = timed(fwd)
+    ms_fb = timed(fwd_bwd)
+    na = int(tgt.shape[1])
+    return {"workload": "%d signals N=%d (chirp + tone + noise), GMW(3,60), nv=32 "
+                        "(na=%d); loss mean(||Tx| - |Tx_target||^2)" % (B, N, na),
+            "ms_forward": round(ms_f, 4), "ms_fwd_bwd": round(ms_fb, 4),
+            "ratio_to_forward": round(ms_fb / ms_f, 3),
+            "value_forward": round(B * N * na / (ms_f * 1e-3) / 1e9, 3),
+            "note": "ms_fwd_bwd includes the loss and its torch backward"}
+
+
 def _traffic():
     """HBM bytes per k_squeeze launch measured with rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy), kept in
@@ -294,6 +335,7 @@ def main():
     }
     if fb is not None:
         res["fwd_bwd"] = fb
+        res["fwd_bwd_batch"] = fwd_bwd_batch(torch, sq)
     if not args.no_batched:
         res["batched_uneven"] = batched_uneven(torch, sq)
     if args.autotune:

@@ -137,6 +137,30 @@ int ssqk_cwt_exec(ssqk_cwt_plan plan, const float* x, int64_t batch,
                   const ssqk_bin_params* bp, double gamma, int gamma_strict,
                   void* workspace, void* stream);
 
+/* One signal of a ragged batch: signals of different lengths that share a
+ * padded length L (and so the scales, frequencies and plan).  x_off: first
+ * sample in x; out_off: first element of its [na, N] Wx / bins block;
+ * tx_off: first element of its [nf, N] Tx block (element offsets). */
+typedef struct ssqk_signal {
+  int64_t N;
+  int64_t n1;
+  int64_t x_off;
+  int64_t out_off;
+  int64_t tx_off;
+} ssqk_signal;
+
+/* Forward CWT and squeeze rows of a ragged batch in one pass: every kernel
+ * of the plan is launched once for all `batch` signals.  sig is a host array
+ * [batch]; each signal needs N <= the plan's N and n1 + N <= L (its own
+ * p2up padding, utils/common.py:32-51).  Outputs as ssqk_cwt_exec without
+ * dWx and w.  workspace: ssqk_cwt_ragged_workspace_size bytes.  Replaces a
+ * loop of cwt() calls over signals of different lengths (the reference
+ * batches only equal lengths, _cwt.py:167-177); no reference counterpart. */
+int ssqk_cwt_ragged_workspace_size(ssqk_cwt_plan plan, int64_t batch, size_t* bytes);
+int ssqk_cwt_exec_ragged(ssqk_cwt_plan plan, const float* x, const ssqk_signal* sig,
+                         int64_t batch, void* Wx, uint16_t* bins, const ssqk_bin_params* bp,
+                         double gamma, int gamma_strict, void* workspace, void* stream);
+
 /* ------------------------------------------------------------- squeeze
  * Tx[b, k, j] = sum over i (ascending) of Wx[b, i, j] * cst[i] for
  * bins[b, i, j] == k.  Deterministic: each column's sum is formed in one
@@ -146,6 +170,13 @@ int ssqk_cwt_exec(ssqk_cwt_plan plan, const float* x, int64_t batch,
 int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* cst,
                  int64_t batch, int64_t na, int64_t nf, int64_t N, void* Tx,
                  void* stream);
+
+/* ssqk_squeeze over a ragged batch (host array sig [batch], offsets as for
+ * ssqk_cwt_exec_ragged), in one launch; workspace: batch * 64 bytes of
+ * device memory.  No reference counterpart (see ssqk_cwt_exec_ragged). */
+int ssqk_squeeze_ragged(const void* Wx, const uint16_t* bins, const float* cst,
+                        const ssqk_signal* sig, int64_t batch, int64_t na, int64_t nf,
+                        void* Tx, void* workspace, void* stream);
 
 /* bins from (Wx, dWx) with the fused phase transform; Sfs (float32 [na])
  * non-NULL selects the STFT form |Sfs[i] - Im(dWx/Wx)/2pi| (algos.py:956-984).

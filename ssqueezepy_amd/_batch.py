@@ -1,16 +1,19 @@
-"""ssq_cwt over many signals of uneven length in one queued pass.
+"""ssq_cwt over many signals of uneven length in a few grouped launches.
 
 The reference takes a batch only as a 2D array of equal-length signals
 (_ssq_cwt.py:12-310 broadcasts over the leading axis); signals of different
 lengths need one call each, and at N ~ 4k-64k each call is a few dozen small
-kernels that leave most of the GPU idle.  ``ssq_cwt_batch`` keeps every
-signal's own grid (scales and ssq_freqs from its length, as the reference
-computes them) and plan, preallocates all inputs, outputs and workspaces,
-and queues the signals' transforms round robin on several streams so that
-their kernels run concurrently.  Runners are cached per (lengths, wavelet,
-options); a new call with the same lengths copies the signals into the
-runner's input buffers and queues the pass again.  Outputs of a pass
-overwrite the previous pass's tensors unless ``copy=True``.
+kernels.  The scales, ssq_freqs and squeeze weights of a signal depend on
+its length only through the padded length L = p2up(N) (utils/cwt_utils.py:
+177, 541), so ``ssq_cwt_batch`` groups the signals by grid: each group shares
+one plan (built for its longest signal) and runs as a ragged batch through
+``ssqk_cwt_exec_ragged`` and ``ssqk_squeeze_ragged`` -- every kernel of the
+plan launched once for the whole group, each signal with its own length,
+padding and output block.  Groups go round robin over several streams.
+Runners (plans, packed buffers, descriptor tables) are cached per (lengths,
+wavelet, options); a new call with the same lengths packs the signals into
+the runner's input buffer and queues the pass again.  Outputs are views of
+packed buffers that the next pass overwrites unless ``copy=True``.
 
 ``graph=True`` (capturing the pass into a HIP graph) is switched off: its
 first run ended the process with a segmentation fault inside the runtime's
@@ -31,53 +34,117 @@ from .wavelets import Wavelet
 __all__ = ['ssq_cwt_batch']
 
 _runners = OrderedDict()
+MAX_GROUP_WORKSPACE = 4 << 30      # bytes of CWT workspace per grouped launch
+
+
+def _grid_key(g):
+    bp = g['bp']
+    return (g['scales64'].tobytes(), np.asarray(g['ssq_freqs']).tobytes(),
+            np.asarray(g['const']).tobytes(),
+            tuple(getattr(bp, f) for f, _ in bp._fields_), g['dt'])
+
+
+class _Group:
+    """Signals that share a grid and padded length: one plan (for the
+    longest), packed input / output buffers and the ssqk_signal table."""
+
+    def __init__(self, members, lengths, g, padtype, device):
+        self.members = members                 # indices into the caller's list
+        wav = g['wavelet']
+        self.g, self.na = g, g['na']
+        Ns = [lengths[m] for m in members]
+        Nmax = max(Ns)
+        L, n1max, pad = _padding(Nmax, padtype)
+        self.plan = _lib.cwt_plan(device, Nmax, L, n1max, pad, wav.kind,
+                                  _wavelet_kernel_params(wav), g['scales64'], g['dt'])
+        na, tot = self.na, sum(Ns)
+        entries, self.views = [], []
+        x_off = 0
+        for N in Ns:
+            _, n1, _ = _padding(N, padtype)
+            entries.append((N, n1, x_off, na * x_off, na * x_off))
+            self.views.append((x_off, N))
+            x_off += N
+        self.sig = _lib.signal_table(entries)
+        self.x = torch.zeros(tot, dtype=torch.float32, device=device)
+        self.Wx = torch.empty(na * tot, dtype=torch.complex64, device=device)
+        self.Tx = torch.empty(na * tot, dtype=torch.complex64, device=device)
+        self.bins = torch.empty(na * tot, dtype=torch.int16, device=device)
+        self.ws = torch.empty(self.plan.ragged_workspace_bytes(len(Ns)), dtype=torch.uint8,
+                              device=device)
+        self.sq_ws = torch.empty(64 * len(Ns), dtype=torch.uint8, device=device)
+        self.cst = _device_const(g, device)
+        self.scales = device_constant(g['scales'], torch.float32, device)
+        self.freqs = g['ssq_freqs'][::-1]
+
+    def outputs(self, k):
+        off, N = self.views[k]
+        na = self.na
+        return (self.Tx[na * off:na * (off + N)].view(na, N),
+                self.Wx[na * off:na * (off + N)].view(na, N))
+
+    def launch(self, gamma):
+        self.plan.exec_ragged(self.x, self.sig, self.Wx, self.bins, self.g['bp'], gamma,
+                              True, self.ws)
+        _lib.squeeze_ragged(self.Wx, self.bins, self.cst, self.sig, self.na, self.na,
+                            self.Tx, self.sq_ws)
 
 
 class _BatchRunner:
     def __init__(self, lengths, wavelet, scales, nv, fs, t, maprange, padtype, flipud,
                  gamma, nstreams, device):
         self.device = device
-        self.items = []
-        for N in lengths:
+        by_grid = OrderedDict()
+        grids = {}
+        for m, N in enumerate(lengths):
             g = ssq_cwt_grid(N, wavelet, scales, nv, fs, t, None, maprange, padtype,
                              flipud)
-            wav = g['wavelet']
-            L, n1, pad = _padding(N, padtype)
-            plan = _lib.cwt_plan(device, N, L, n1, pad, wav.kind,
-                                 _wavelet_kernel_params(wav), g['scales64'], g['dt'])
-            na = g['na']
-            it = dict(
-                N=N, g=g, plan=plan, na=na,
-                x=torch.zeros(N, dtype=torch.float32, device=device),
-                Wx=torch.empty((na, N), dtype=torch.complex64, device=device),
-                Tx=torch.empty((na, N), dtype=torch.complex64, device=device),
-                bins=torch.empty((na, N), dtype=torch.int16, device=device),
-                ws=torch.empty(plan.workspace_bytes(1), dtype=torch.uint8, device=device),
-                cst=_device_const(g, device),
-                scales=device_constant(g['scales'], torch.float32, device),
-                freqs=g['ssq_freqs'][::-1])
-            self.items.append(it)
+            L = _padding(N, padtype)[0]
+            key = (L, _grid_key(g))
+            by_grid.setdefault(key, []).append(m)
+            grids[key] = g
+        self.groups = []
+        for key, members in by_grid.items():
+            # longest first, split so that each grouped launch's workspace
+            # stays within MAX_GROUP_WORKSPACE
+            members = sorted(members, key=lambda m: -lengths[m])
+            while members:
+                n = len(members)
+                grp = _Group(members[:n], lengths, grids[key], padtype, device)
+                while n > 1 and grp.ws.numel() > MAX_GROUP_WORKSPACE:
+                    n = max(1, n // 2)
+                    grp = _Group(members[:n], lengths, grids[key], padtype, device)
+                self.groups.append(grp)
+                members = members[n:]
+        self.where = {}
+        for gi, grp in enumerate(self.groups):
+            for k, m in enumerate(grp.members):
+                self.where[m] = (gi, k)
         self.gamma = gamma
         self.streams = [torch.cuda.Stream(device=device) for _ in range(nstreams)]
 
     def _launch(self):
-        """Queue every signal's transform and squeeze; the streams fork from
-        and join into the current stream."""
+        """Queue every group's ragged transform and squeeze; the streams fork
+        from and join into the current stream."""
         cur = torch.cuda.current_stream(self.device)
         for s in self.streams:
             s.wait_stream(cur)
-        for k, it in enumerate(self.items):
-            with torch.cuda.stream(self.streams[k % len(self.streams)]):
-                it['plan'].exec(it['x'], it['Wx'], bins=it['bins'], bp=it['g']['bp'],
-                                gamma=self.gamma, strict=True, workspace=it['ws'])
-                _lib.squeeze(it['Wx'], it['bins'], it['cst'], it['na'], it['Tx'])
+        for gi, grp in enumerate(self.groups):
+            with torch.cuda.stream(self.streams[gi % len(self.streams)]):
+                grp.launch(self.gamma)
         for s in self.streams:
             cur.wait_stream(s)
 
     def run(self, xs):
-        for it, x in zip(self.items, xs):
-            it['x'].copy_(x, non_blocking=True)
+        for grp in self.groups:
+            torch.cat([xs[m] for m in grp.members], out=grp.x)
         self._launch()
+
+    def result(self, m):
+        gi, k = self.where[m]
+        grp = self.groups[gi]
+        Tx, Wx = grp.outputs(k)
+        return Tx, Wx, grp
 
 
 def _as_signal(x, device):
@@ -93,9 +160,13 @@ def ssq_cwt_batch(xs, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t
                   astensor=True, graph=False, streams=4, copy=False):
     """Synchrosqueezed CWT of each 1D signal in ``xs`` (any lengths).
 
-    Returns a list of (Tx, Wx, ssq_freqs, scales), one per signal, equal to
+    Returns a list of (Tx, Wx, ssq_freqs, scales), one per signal: the
     ``ssq_cwt(x, wavelet, scales, nv, fs, t, padtype=padtype, maprange=maprange,
-    flipud=flipud, gamma=gamma)`` of each signal (same plans and kernels).
+    flipud=flipud, gamma=gamma)`` of each signal, computed with the plan of the
+    longest signal of its group.  ssq_freqs and scales are those of ssq_cwt;
+    Tx and Wx equal ssq_cwt's bit for bit for the longest signal of each
+    group, and to float32 rounding for the others (a plan picks each scale's
+    path by a cost that depends on N; the paths compute the same transform).
     """
     if not torch.cuda.is_available():
         raise RuntimeError("ssqueezepy_amd needs a ROCm GPU")
@@ -124,11 +195,10 @@ def ssq_cwt_batch(xs, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t
         _runners.move_to_end(key)
     runner.run([_as_signal(x, device) for x in xs])
     out = []
-    for it in runner.items:
-        Tx, Wx = it['Tx'], it['Wx']
-        sc = it['scales'].clone()
+    for m in range(len(xs)):
+        Tx, Wx, grp = runner.result(m)
         if copy:
             Tx, Wx = Tx.clone(), Wx.clone()
-        out.append((to_output(Tx, astensor), to_output(Wx, astensor), it['freqs'],
-                    to_output(sc, astensor) if astensor else it['g']['scales'].copy()))
+        sc = grp.scales.clone() if astensor else grp.g['scales'].copy()
+        out.append((to_output(Tx, astensor), to_output(Wx, astensor), grp.freqs, sc))
     return out

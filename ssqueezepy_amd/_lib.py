@@ -40,6 +40,20 @@ def plan_opts(side_streams=-1, os_group=0, os_bias=0.):
     return PlanOpts(int(side_streams), int(os_group), float(os_bias))
 
 
+class Signal(ctypes.Structure):
+    """ssqk_signal: one signal of a ragged batch (element offsets)."""
+    _fields_ = [('N', c_int64), ('n1', c_int64), ('x_off', c_int64),
+                ('out_off', c_int64), ('tx_off', c_int64)]
+
+
+def signal_table(entries):
+    """ctypes array of ssqk_signal from (N, n1, x_off, out_off, tx_off) tuples."""
+    arr = (Signal * len(entries))()
+    for k, e in enumerate(entries):
+        arr[k] = Signal(*[int(v) for v in e])
+    return arr
+
+
 _SIGS = {
     'ssqk_last_error': (ctypes.c_char_p, []),
     'ssqk_version': (c_int, []),
@@ -60,6 +74,14 @@ _SIGS = {
     'ssqk_cwt_exec': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                               c_void_p, c_void_p, ctypes.POINTER(BinParams),
                               c_double, c_int, c_void_p, c_void_p]),
+    'ssqk_cwt_ragged_workspace_size': (c_int, [c_void_p, c_int64,
+                                               ctypes.POINTER(c_size_t)]),
+    'ssqk_cwt_exec_ragged': (c_int, [c_void_p, c_void_p, ctypes.POINTER(Signal), c_int64,
+                                     c_void_p, c_void_p, ctypes.POINTER(BinParams),
+                                     c_double, c_int, c_void_p, c_void_p]),
+    'ssqk_squeeze_ragged': (c_int, [c_void_p, c_void_p, c_void_p, ctypes.POINTER(Signal),
+                                    c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                    c_void_p]),
     'ssqk_cwt_adjoint': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     'ssqk_cwt_adjoint_workspace_size': (c_int, [c_void_p, c_int64, c_int,
@@ -190,6 +212,19 @@ class CwtPlan:
             int(bool(strict)), _ptr(workspace), _stream(x.device)))
         return workspace
 
+    def ragged_workspace_bytes(self, batch):
+        n = c_size_t()
+        check(load().ssqk_cwt_ragged_workspace_size(self._h, int(batch), ctypes.byref(n)))
+        return n.value
+
+    def exec_ragged(self, x, sig, Wx, bins, bp, gamma, strict, workspace):
+        """Forward CWT + squeeze rows of the signals of ``sig`` (a
+        signal_table) packed in the flat device buffers x, Wx, bins."""
+        check(load().ssqk_cwt_exec_ragged(
+            self._h, _ptr(x), sig, len(sig), _ptr(Wx), _ptr(bins),
+            ctypes.byref(bp) if bp is not None else None, float(gamma),
+            int(bool(strict)), _ptr(workspace), _stream(x.device)))
+
     def adjoint(self, gx, gTx=None, gWx=None, bins=None, cst=None, nf=0):
         """gx = d loss / d x through Wx (gWx) and Tx (gTx, with the forward's
         squeeze rows ``bins`` and weights ``cst`` held fixed)."""
@@ -251,6 +286,12 @@ def cwt_plan(device, N, L, n1, pad, kind, params, scales64, dt):
 
 
 # ---------------------------------------------------------- thin wrappers
+def squeeze_ragged(Wx, bins, cst, sig, na, nf, Tx, workspace):
+    """ssqk_squeeze over the signals of ``sig`` packed in flat buffers;
+    workspace: >= 64 * len(sig) bytes of device memory."""
+    check(load().ssqk_squeeze_ragged(_ptr(Wx), _ptr(bins), _ptr(cst), sig, len(sig), int(na),
+                                     int(nf), _ptr(Tx), _ptr(workspace), _stream(Wx.device)))
+
 def squeeze(Wx, bins, cst, nf, Tx):
     batch = 1 if Wx.ndim == 2 else Wx.shape[0]
     na, N = Wx.shape[-2], Wx.shape[-1]

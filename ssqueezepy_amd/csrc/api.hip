@@ -3,6 +3,7 @@
 #include "common.hpp"
 #include "fft_core.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -191,8 +192,9 @@ __device__ __forceinline__ float quad_bcast(float v) {
 template <int W>
 __global__ __launch_bounds__(W * 4 * kSqQPC)
 void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
-               const float* __restrict__ cst, int na, int r0, int R, int64_t N,
-               int64_t in_stride, int64_t out_stride, float2* __restrict__ Tx) {
+               const float* __restrict__ cst, int na, int r0, int R, int64_t Ngrid,
+               int64_t in_stride, int64_t out_stride, float2* __restrict__ Tx,
+               const SigDev* __restrict__ sig) {
   constexpr int NT = W * 4 * kSqQPC;
   constexpr int RS = W + 1;
   extern __shared__ float2 s_acc[];          // [R + 4][RS]
@@ -202,12 +204,20 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
   // tiles share is fetched into one L2 once
   int tile = blockIdx.x;
   if (kSqXcd) {
-    const int ntiles = (int)((N + W - 1) / W);
+    const int ntiles = (int)((Ngrid + W - 1) / W);
     const int x = tile & 7, jx = tile >> 3, qx = ntiles >> 3, rx = ntiles & 7;
     if (jx >= qx + (x < rx ? 1 : 0)) return;
     tile = x * qx + (x < rx ? x : rx) + jx;
   }
+  // signal b: uniform layout, or its entry of a ragged batch
+  int64_t N = Ngrid, ioff = (int64_t)b * in_stride, toff = (int64_t)b * out_stride;
+  if (sig) {
+    N = sig[b].N;
+    ioff = sig[b].out_off;
+    toff = sig[b].tx_off;
+  }
   const int64_t j0 = (int64_t)tile * W;
+  if (j0 >= N) return;                       // (shorter signal of a ragged batch)
   const int tid = threadIdx.x;
   for (int e = tid; e < (R + 4) * RS; e += NT) s_acc[e] = make_float2(0.f, 0.f);
   __syncthreads();
@@ -220,8 +230,8 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
   const bool live = j0 + c < N;
   const uint32_t cl = live ? (uint32_t)c : 0u;
   const uint32_t offL = (uint32_t)sl * (uint32_t)N + cl;
-  const float2* Wb = Wx + (int64_t)b * in_stride + j0;
-  const uint16_t* Bb = bins + (int64_t)b * in_stride + j0;
+  const float2* Wb = Wx + ioff + j0;
+  const uint16_t* Bb = bins + ioff + j0;
   const int dummy = R + q;
 
   // rows past na re-read row ib (in bounds) and are masked below
@@ -289,7 +299,7 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
     load(i0 + 3 * BS, wB, kB, cB);
   }
   __syncthreads();
-  float2* tp = Tx + (int64_t)b * out_stride + (int64_t)r0 * N;
+  float2* tp = Tx + toff + (int64_t)r0 * N;
   for (int e = tid; e < R * W; e += NT) {
     const int k = e / W, cc = e - k * W;
     const int64_t jj = j0 + cc;
@@ -387,14 +397,11 @@ extern "C" int ssqk_bins_from_w(const float* w, int64_t rows, int64_t N,
   return SSQK_OK;
 }
 
-extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* cst,
-                            int64_t batch, int64_t na, int64_t nf, int64_t N, void* Tx,
-                            void* stream) {
-  if (!Wx || !bins || !cst || !Tx || batch < 1 || batch > 65535 || na < 1 || nf < 1 ||
-      nf >= kNoBin || N < 1 || N * kSqGS >= ((int64_t)1 << 31)) {
-    ssqk_host::set_error("bad arguments to ssqk_squeeze");
-    return SSQK_ERR_ARG;
-  }
+namespace {
+// N: columns of the grid (the longest signal of a ragged batch)
+int squeeze_impl(const void* Wx, const uint16_t* bins, const float* cst, int64_t batch,
+                 int64_t na, int64_t nf, int64_t N, void* Tx, const SigDev* sig,
+                 void* stream) {
   // accumulators [R + 4][W + 1] complex in 160 KiB of LDS: 64 columns while
   // all rows fit (nf <= 311), else 32, 16 or 8; taller outputs are done in
   // row windows
@@ -419,7 +426,7 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
       dim3 grid(kSqXcd ? (ntiles + 7) / 8 * 8 : ntiles, (unsigned)batch);
       hipLaunchKernelGGL(kern, grid, dim3(W * 4 * kSqQPC), lds, (hipStream_t)stream,
                          (const float2*)Wx, bins, cst, (int)na, (int)r0, (int)rows, N,
-                         na * N, nf * N, (float2*)Tx);
+                         na * N, nf * N, (float2*)Tx, sig);
       SSQK_CHECK(hipGetLastError());
     }
     return SSQK_OK;
@@ -428,4 +435,41 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
   if (W == 32) return launch(k_squeeze<32>, 1);
   if (W == 16) return launch(k_squeeze<16>, 2);
   return launch(k_squeeze<8>, 3);
+}
+}  // namespace
+
+extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* cst,
+                            int64_t batch, int64_t na, int64_t nf, int64_t N, void* Tx,
+                            void* stream) {
+  if (!Wx || !bins || !cst || !Tx || batch < 1 || batch > 65535 || na < 1 || nf < 1 ||
+      nf >= kNoBin || N < 1 || N * kSqGS >= ((int64_t)1 << 31)) {
+    ssqk_host::set_error("bad arguments to ssqk_squeeze");
+    return SSQK_ERR_ARG;
+  }
+  return squeeze_impl(Wx, bins, cst, batch, na, nf, N, Tx, nullptr, stream);
+}
+
+extern "C" int ssqk_squeeze_ragged(const void* Wx, const uint16_t* bins, const float* cst,
+                                   const ssqk_signal* sig, int64_t batch, int64_t na,
+                                   int64_t nf, void* Tx, void* workspace, void* stream) {
+  if (!Wx || !bins || !cst || !Tx || !sig || !workspace || batch < 1 || batch > 65535 ||
+      na < 1 || nf < 1 || nf >= kNoBin) {
+    ssqk_host::set_error("bad arguments to ssqk_squeeze_ragged");
+    return SSQK_ERR_ARG;
+  }
+  std::vector<SigDev> d((size_t)batch);
+  int64_t nmax = 0;
+  for (int64_t b = 0; b < batch; ++b) {
+    const ssqk_signal& g = sig[b];
+    if (g.N < 1 || g.N * kSqGS >= ((int64_t)1 << 31) || g.out_off < 0 || g.tx_off < 0) {
+      ssqk_host::set_error("signal %lld: bad length or offsets", (long long)b);
+      return SSQK_ERR_ARG;
+    }
+    d[b] = SigDev{(int)g.N, (int)g.n1, g.x_off, g.out_off, g.tx_off};
+    nmax = std::max<int64_t>(nmax, g.N);
+  }
+  SSQK_CHECK(hipMemcpyAsync(workspace, d.data(), d.size() * sizeof(SigDev),
+                            hipMemcpyHostToDevice, (hipStream_t)stream));
+  return squeeze_impl(Wx, bins, cst, batch, na, nf, nmax, Tx, (const SigDev*)workspace,
+                      stream);
 }

@@ -28,6 +28,14 @@ __device__ __forceinline__ double psih_eval(const WavDev& p, double w) {
   return p.d * (exp(p.c * u * u) - p.b * exp(p.c * w * w));
 }
 
+// One signal of a ragged batch (signals of different lengths that share a
+// padded length): its length, left pad, and where its input, its [na, N]
+// Wx / bins block and its [nf, N] Tx block start (in elements).
+struct SigDev {
+  int N, n1;
+  int64_t x_off, out_off, tx_off;
+};
+
 // Squeeze bin parameters (algos.py:44-123, 356-374).  type: 0 linear,
 // 1 log, 2 log-piecewise.  For log types v0/v1 are log2 of the grid's
 // first (and transition) frequency and d0/d1 the log2 steps.

@@ -65,7 +65,22 @@ struct CwtArgs {
   int pS;
   const float* psih;        // psih(s_i xi_k) table, scale i at psih_off[i] + (k - ka_i)
   const int64_t* psih_off;
+  const SigDev* sig;        // ragged batch: per-signal N, n1, offsets (else null)
 };
+
+// Length, left pad and offsets of signal b: from the ragged-batch table, or
+// the uniform layout [batch][N] / [batch][na][N].
+struct Sig {
+  int N, n1;
+  int64_t x_off, out_off;
+};
+__device__ __forceinline__ Sig sig_at(const CwtArgs& a, int b) {
+  if (a.sig) {
+    const SigDev& d = a.sig[b];
+    return Sig{d.N, d.n1, d.x_off, d.out_off};
+  }
+  return Sig{a.N, a.n1, (int64_t)b * a.N, (int64_t)b * a.out_stride};
+}
 
 // XCD-aware task order.  Workgroups are dispatched to the 8 XCDs round
 // robin (block b to XCD b % 8), each XCD with its own L2.  Block b takes
@@ -115,11 +130,11 @@ __device__ __forceinline__ int64_t pad_src(int64_t n, int64_t N, int64_t n1, int
 // false when the point's bin lies at a float32 rounding edge: the caller
 // then finishes the point with emit_edge (the float64 evaluation, long code
 // kept out of unrolled loops).
-__device__ __forceinline__ bool emit_fast(const CwtArgs& a, int b, int i, int64_t n,
+__device__ __forceinline__ bool emit_fast(const CwtArgs& a, const Sig& s, int i, int64_t n,
                                           float2 z, float2 zd) {
-  const int64_t j = n - a.n1;
-  if (j < 0 || j >= a.N) return true;
-  const int64_t o = (int64_t)b * a.out_stride + (int64_t)i * a.N + j;
+  const int64_t j = n - s.n1;
+  if (j < 0 || j >= s.N) return true;
+  const int64_t o = s.out_off + (int64_t)i * s.N + j;
   a.Wx[o] = z;
   if (a.dWx) a.dWx[o] = zd;
   const float m2 = z.x * z.x + z.y * z.y;
@@ -142,9 +157,9 @@ __device__ __forceinline__ bool emit_fast(const CwtArgs& a, int b, int i, int64_
   return true;
 }
 
-__device__ __forceinline__ void emit_edge(const CwtArgs& a, int b, int i, int64_t n,
+__device__ __forceinline__ void emit_edge(const CwtArgs& a, const Sig& s, int i, int64_t n,
                                           float2 z, float2 zd) {
-  const int64_t o = (int64_t)b * a.out_stride + (int64_t)i * a.N + (n - a.n1);
+  const int64_t o = s.out_off + (int64_t)i * s.N + (n - s.n1);
   int k;
   if (a.strict) {
     k = bin_edge_dwx(zd.x, zd.y, z.x, z.y, a.bp);
@@ -155,9 +170,9 @@ __device__ __forceinline__ void emit_edge(const CwtArgs& a, int b, int i, int64_
   a.bins[o] = (uint16_t)k;
 }
 
-__device__ __forceinline__ void emit(const CwtArgs& a, int b, int i, int64_t n,
+__device__ __forceinline__ void emit(const CwtArgs& a, const Sig& s, int i, int64_t n,
                                      float2 z, float2 zd) {
-  if (!emit_fast(a, b, i, n, z, zd)) emit_edge(a, b, i, n, z, zd);
+  if (!emit_fast(a, s, i, n, z, zd)) emit_edge(a, s, i, n, z, zd);
 }
 
 // X = psih(s xi_k) v and Xd = X * i xi_k / dt for a positive bin k, where
@@ -217,6 +232,7 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
   const float2* xh = a.xh + (int64_t)b * a.xh_stride;
   const float* ps = a.psih + a.psih_off[i];
   const float invL = 1.0f / (float)L;   // a power of two: scaling is exact
+  const Sig sg = sig_at(a, b);
 
   // band samples into row 0 at their cyclic positions (the table is zero
   // past the Nyquist bin)
@@ -286,8 +302,8 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
     const int r = e % ROWS, q = e / ROWS;
     const int ph_a = a0 + r;
     const int64_t n = ph_a + (int64_t)D * q;
-    if (e < ROWS * M && ph_a < D && n >= a.n1 && n < (int64_t)a.n1 + a.N &&
-        !emit_fast(a, b, i, n, sW[Lds<M>::idx(r, q)], sD[Lds<M>::idx(r, q)]))
+    if (e < ROWS * M && ph_a < D && n >= sg.n1 && n < (int64_t)sg.n1 + sg.N &&
+        !emit_fast(a, sg, i, n, sW[Lds<M>::idx(r, q)], sD[Lds<M>::idx(r, q)]))
       edge |= 1u << it;
   }
   while (edge) {
@@ -295,7 +311,7 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
     edge &= edge - 1;
     const int e = threadIdx.x + it * kNT;
     const int r = e % ROWS, q = e / ROWS;
-    emit_edge(a, b, i, a0 + r + (int64_t)D * q, sW[Lds<M>::idx(r, q)],
+    emit_edge(a, sg, i, a0 + r + (int64_t)D * q, sW[Lds<M>::idx(r, q)],
               sD[Lds<M>::idx(r, q)]);
   }
 }
@@ -410,13 +426,14 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullB(CwtArgs a, const int4* __
   float2* const bufs[2] = {sW, sD};
   lds_fft<L1, G, +1, kNT, 2>(bufs, tw);
   const float invL = 1.0f / (float)a.L;
+  const Sig sg = sig_at(a, b);
   for (int e = threadIdx.x; e < G * L1; e += kNT) {
     const int g = e % G, n1 = e / G;
     const int64_t n = (int64_t)L2 * n1 + n2_0 + g;
-    if (n < a.n1 || n >= (int64_t)a.n1 + a.N) continue;
+    if (n < sg.n1 || n >= (int64_t)sg.n1 + sg.N) continue;
     float2 z = sW[Lds<L1>::idx(g, n1)], zd = sD[Lds<L1>::idx(g, n1)];
     z.x *= invL; z.y *= invL; zd.x *= invL; zd.y *= invL;
-    emit(a, b, i, n, z, zd);
+    emit(a, sg, i, n, z, zd);
   }
 }
 
@@ -447,7 +464,7 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
                                                  int coef_stride,
                                                  const int2* __restrict__ coef_band,
                                                  const float2* __restrict__ tw,
-                                                 const float* __restrict__ x, int64_t x_stride,
+                                                 const float* __restrict__ x,
                                                  int pad) {
   constexpr int SR = OsShape<F>::SR;
   constexpr int NT = OsShape<F>::NT;
@@ -461,8 +478,10 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
   const int b = blockIdx.y;
   const int64_t L = a.L;
   const int64_t j_base = (int64_t)t.y * g.B;              // first unpadded output
-  const int64_t start = (int64_t)a.n1 + j_base - g.H;     // padded index of seg[0]
-  const float* xb = x + (int64_t)b * x_stride;
+  const Sig sg = sig_at(a, b);
+  if (j_base >= sg.N) return;                             // (shorter ragged signal)
+  const int64_t start = (int64_t)sg.n1 + j_base - g.H;    // padded index of seg[0]
+  const float* xb = x + sg.x_off;
   constexpr int IS = (F + NT - 1) / NT;
   {
     float xv[IS];
@@ -471,7 +490,7 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
       const int e = threadIdx.x + it * NT;
       int64_t n = (start + e) % L;
       if (n < 0) n += L;
-      const int64_t src = pad_src(n, a.N, a.n1, pad);
+      const int64_t src = pad_src(n, sg.N, sg.n1, pad);
       xv[it] = xb[src >= 0 ? src : 0] * (src >= 0 ? 1.f : 0.f);
     }
 #pragma unroll
@@ -518,9 +537,9 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
       const int r = e / g.B, tt = e - r * g.B;
       if (s0 + r >= g.ns) continue;
       const int64_t j = j_base + tt;
-      if (j >= a.N) continue;
+      if (j >= sg.N) continue;
       const int pos = g.H + tt;
-      emit(a, b, g.i0 + s0 + r, (int64_t)a.n1 + j, sA[Lds<F>::idx(r, pos)],
+      emit(a, sg, g.i0 + s0 + r, (int64_t)sg.n1 + j, sA[Lds<F>::idx(r, pos)],
            sD[Lds<F>::idx(r, pos)]);
     }
     __syncthreads();
@@ -574,16 +593,30 @@ struct FwdArgs {
   int64_t xh_stride;
   const float2* ptab;
   int pS;
+  const SigDev* sig;        // ragged batch (else null)
 };
+
+// input, length and left pad of signal b
+struct FwdSig {
+  const float* x;
+  int64_t N, n1;
+};
+__device__ __forceinline__ FwdSig fwd_sig(const FwdArgs& f, int b) {
+  if (f.sig) {
+    const SigDev& d = f.sig[b];
+    return FwdSig{f.x + d.x_off, d.N, d.n1};
+  }
+  return FwdSig{f.x + (int64_t)b * f.N, f.N, f.n1};
+}
 
 template <int L>
 __global__ __launch_bounds__(kNT) void k_fwd_small(FwdArgs f, const float2* __restrict__ tw) {
   __shared__ float2 s[Lds<L>::elems(1)];
   const int b = blockIdx.x;
-  const float* x = f.x + (int64_t)b * f.N;
+  const FwdSig fs = fwd_sig(f, b);
   for (int n = threadIdx.x; n < L; n += kNT) {
-    const int64_t src = pad_src(n, f.N, f.n1, f.pad);
-    s[Lds<L>::idx(0, n)] = make_float2(src >= 0 ? x[src] : 0.f, 0.f);
+    const int64_t src = pad_src(n, fs.N, fs.n1, f.pad);
+    s[Lds<L>::idx(0, n)] = make_float2(src >= 0 ? fs.x[src] : 0.f, 0.f);
   }
   float2* const bufs[1] = {s};
   lds_fft<L, 1, -1, kNT, 1>(bufs, tw);
@@ -598,11 +631,11 @@ __global__ __launch_bounds__(kNT) void k_fwd_A(FwdArgs f, const float2* __restri
   __shared__ float2 s[Lds<L2>::elems(G)];
   const int b = blockIdx.y;
   const int k1_0 = blockIdx.x * G;
-  const float* x = f.x + (int64_t)b * f.N;
+  const FwdSig fs = fwd_sig(f, b);
   for (int e = threadIdx.x; e < G * L2; e += kNT) {
     const int g = e % G, k2 = e / G;
-    const int64_t src = pad_src(k1_0 + g + (int64_t)L1 * k2, f.N, f.n1, f.pad);
-    s[Lds<L2>::idx(g, k2)] = make_float2(src >= 0 ? x[src] : 0.f, 0.f);
+    const int64_t src = pad_src(k1_0 + g + (int64_t)L1 * k2, fs.N, fs.n1, f.pad);
+    s[Lds<L2>::idx(g, k2)] = make_float2(src >= 0 ? fs.x[src] : 0.f, 0.f);
   }
   float2* const bufs[1] = {s};
   lds_fft<L2, G, -1, kNT, 1>(bufs, tw);
@@ -700,13 +733,13 @@ int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
 
 int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup* groups,
               const int2* tasks, const float2* coef, int coef_stride, const int2* coef_band,
-              const float* x, int64_t x_stride, int pad) {
+              const float* x, int pad) {
   SSQK_GET_TW(tw, F);
   switch (F) {
 #define C(m)                                                                        \
   case m:                                                                           \
     hipLaunchKernelGGL(k_cwt_os<m>, grid, dim3(OsShape<m>::NT), 0, st, a, groups,   \
-                       tasks, coef, coef_stride, coef_band, tw, x, x_stride, pad);  \
+                       tasks, coef, coef_stride, coef_band, tw, x, pad);            \
     break;
     SSQK_OS_CASES(C)
 #undef C
@@ -992,7 +1025,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
                      double dt, double band_rel, PlanOpts opt);
 int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void* dWx,
               float* w, uint16_t* bins, const ssqk_bin_params* bp, double gamma,
-              int gamma_strict, void* workspace, hipStream_t st);
+              int gamma_strict, void* workspace, hipStream_t st, const SigDev* sig);
 
 // Kernel spectra of the overlap-save scales: an internal plan (band and
 // four-step paths only) computes Wx and dWx of a unit impulse centred in a
@@ -1025,7 +1058,8 @@ int build_os_coefficients(ssqk_cwt_plan_s* p, const std::vector<int>& os_scales,
   if (e == hipSuccess) e = hipMalloc(&d_hD, sizeof(float2) * hN * s_os.size());
   if (e == hipSuccess) e = hipMalloc(&d_ws, wsz);
   if (e == hipSuccess) {
-    rc = exec_impl(imp, d_x, 1, d_hW, d_hD, nullptr, nullptr, nullptr, 0.0, 1, d_ws, nullptr);
+    rc = exec_impl(imp, d_x, 1, d_hW, d_hD, nullptr, nullptr, nullptr, 0.0, 1, d_ws, nullptr,
+                   nullptr);
   }
   OsGroup* d_groups_all = nullptr;
   if (e == hipSuccess && rc == 0) e = upload(&d_groups_all, all_groups);
@@ -1565,7 +1599,7 @@ extern "C" int ssqk_cwt_workspace_size(ssqk_cwt_plan p, int64_t batch, size_t* b
 namespace {
 int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void* dWx,
               float* w, uint16_t* bins, const ssqk_bin_params* bp, double gamma,
-              int gamma_strict, void* workspace, hipStream_t st) {
+              int gamma_strict, void* workspace, hipStream_t st, const SigDev* sig) {
   size_t off_fwd, off_full, total;
   ws_layout(p, batch, &off_fwd, &off_full, &total);
   char* ws = (char*)workspace;
@@ -1587,6 +1621,7 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
   a.pS = phase_tab_split(p->L);
   a.psih = p->d_psih;
   a.psih_off = p->d_psih_off;
+  a.sig = sig;
   if (!a.ptab) {
     ssqk_host::set_error("could not create the phase table");
     return SSQK_ERR_HIP;
@@ -1642,6 +1677,7 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     f.xh = xh; f.xh_stride = xh_stride(p);
     f.ptab = a.ptab;
     f.pS = a.pS;
+    f.sig = sig;
     rc = launch_fwd(f, (int)batch, st, p->fL1, p->fL2, (float2*)(ws + off_fwd));
     if (rc) return rc;
     SSQK_CHECK(hipGetLastError());
@@ -1651,7 +1687,7 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     if (items[t].kind != 0) continue;
     const auto& oc = p->os[items[t].idx];
     rc = launch_os(oc.F, dim3(oc.ntasks, (unsigned)batch), streams[where[t]], a, oc.d_groups,
-                   oc.d_tasks, p->d_coef, p->coef_stride, p->d_coef_band, x, p->N, p->pad);
+                   oc.d_tasks, p->d_coef, p->coef_stride, p->d_coef_band, x, p->pad);
     if (rc) return rc;
     SSQK_CHECK(hipGetLastError());
   }
@@ -1711,7 +1747,62 @@ extern "C" int ssqk_cwt_exec(ssqk_cwt_plan p, const float* x, int64_t batch, voi
     return SSQK_ERR_ARG;
   }
   return exec_impl(p, x, batch, Wx, dWx, w, bins, bp, gamma, gamma_strict, workspace,
-                   (hipStream_t)stream);
+                   (hipStream_t)stream, nullptr);
+}
+
+namespace {
+size_t ragged_sig_offset(ssqk_cwt_plan_s* p, int64_t batch) {
+  size_t a, b, total;
+  ws_layout(p, batch, &a, &b, &total);
+  return (total + 255) / 256 * 256;
+}
+}  // namespace
+
+extern "C" int ssqk_cwt_ragged_workspace_size(ssqk_cwt_plan p, int64_t batch, size_t* bytes) {
+  if (!p || !bytes || batch < 1) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_ragged_workspace_size");
+    return SSQK_ERR_ARG;
+  }
+  *bytes = ragged_sig_offset(p, batch) + (size_t)batch * sizeof(SigDev);
+  return SSQK_OK;
+}
+
+extern "C" int ssqk_cwt_exec_ragged(ssqk_cwt_plan p, const float* x, const ssqk_signal* sig,
+                                    int64_t batch, void* Wx, uint16_t* bins,
+                                    const ssqk_bin_params* bp, double gamma, int gamma_strict,
+                                    void* workspace, void* stream) {
+  if (!p || !x || !sig || !Wx || !workspace || batch < 1 || batch > 65535) {
+    ssqk_host::set_error("bad arguments to ssqk_cwt_exec_ragged");
+    return SSQK_ERR_ARG;
+  }
+  if (bins && !bp) {
+    ssqk_host::set_error("bins requested without bin parameters");
+    return SSQK_ERR_ARG;
+  }
+  if (bins && bp && (bp->omax < 0 || bp->omax >= kNoBin)) {
+    ssqk_host::set_error("omax=%lld out of range for uint16 bins", (long long)bp->omax);
+    return SSQK_ERR_ARG;
+  }
+  // every signal must fit the plan: the overlap-save blocks cover the
+  // plan's N outputs and the padded length is the plan's L
+  std::vector<SigDev> d((size_t)batch);
+  for (int64_t b = 0; b < batch; ++b) {
+    const ssqk_signal& g = sig[b];
+    if (g.N < 1 || g.N > p->N || g.n1 < 0 || g.n1 + g.N > p->L || g.x_off < 0 ||
+        g.out_off < 0 || g.tx_off < 0) {
+      ssqk_host::set_error("signal %lld: N=%lld n1=%lld does not fit the plan (N <= %lld, "
+                           "n1 + N <= L=%lld)", (long long)b, (long long)g.N,
+                           (long long)g.n1, (long long)p->N, (long long)p->L);
+      return SSQK_ERR_ARG;
+    }
+    d[b] = SigDev{(int)g.N, (int)g.n1, g.x_off, g.out_off, g.tx_off};
+  }
+  char* ws = (char*)workspace;
+  SigDev* dsig = (SigDev*)(ws + ragged_sig_offset(p, batch));
+  SSQK_CHECK(hipMemcpyAsync(dsig, d.data(), d.size() * sizeof(SigDev), hipMemcpyHostToDevice,
+                            (hipStream_t)stream));
+  return exec_impl(p, x, batch, Wx, nullptr, nullptr, bins, bp, gamma, gamma_strict,
+                   workspace, (hipStream_t)stream, dsig);
 }
 
 // =========================================================== adjoint
