@@ -9,11 +9,12 @@ its length only through the padded length L = p2up(N) (utils/cwt_utils.py:
 one plan (built for its longest signal) and runs as a ragged batch through
 ``ssqk_cwt_exec_ragged`` and ``ssqk_squeeze_ragged`` -- every kernel of the
 plan launched once for the whole group, each signal with its own length,
-padding and output block.  Groups go round robin over several streams.
+padding and output block.
 Runners (plans, packed buffers, descriptor tables) are cached per (lengths,
 wavelet, options); a new call with the same lengths packs the signals into
 the runner's input buffer and queues the pass again.  Outputs are views of
-packed buffers that the next pass overwrites unless ``copy=True``.
+packed buffers that the next pass overwrites unless ``copy=True``.  Groups
+are spread over the streams largest first, each to the least loaded one.
 
 ``graph=True`` (capturing the pass into a HIP graph) is switched off: its
 first run ended the process with a segmentation fault inside the runtime's
@@ -58,6 +59,7 @@ class _Group:
         self.plan = _lib.cwt_plan(device, Nmax, L, n1max, pad, wav.kind,
                                   _wavelet_kernel_params(wav), g['scales64'], g['dt'])
         na, tot = self.na, sum(Ns)
+        self.work = float(na) * tot
         entries, self.views = [], []
         x_off = 0
         for N in Ns:
@@ -65,7 +67,7 @@ class _Group:
             entries.append((N, n1, x_off, na * x_off, na * x_off))
             self.views.append((x_off, N))
             x_off += N
-        self.sig = _lib.signal_table(entries)
+        self.sig = _lib.signal_table(entries, pinned=True)
         self.x = torch.zeros(tot, dtype=torch.float32, device=device)
         self.Wx = torch.empty(na * tot, dtype=torch.complex64, device=device)
         self.Tx = torch.empty(na * tot, dtype=torch.complex64, device=device)
@@ -122,6 +124,13 @@ class _BatchRunner:
                 self.where[m] = (gi, k)
         self.gamma = gamma
         self.streams = [torch.cuda.Stream(device=device) for _ in range(nstreams)]
+        # largest group first, each to the least loaded stream (work ~ na * sum N)
+        load = [0.0] * nstreams
+        self.order = []
+        for gi in sorted(range(len(self.groups)), key=lambda gi: -self.groups[gi].work):
+            s = min(range(nstreams), key=lambda k: load[k])
+            load[s] += self.groups[gi].work
+            self.order.append((gi, s))
 
     def _launch(self):
         """Queue every group's ragged transform and squeeze; the streams fork
@@ -129,9 +138,9 @@ class _BatchRunner:
         cur = torch.cuda.current_stream(self.device)
         for s in self.streams:
             s.wait_stream(cur)
-        for gi, grp in enumerate(self.groups):
-            with torch.cuda.stream(self.streams[gi % len(self.streams)]):
-                grp.launch(self.gamma)
+        for gi, si in self.order:
+            with torch.cuda.stream(self.streams[si]):
+                self.groups[gi].launch(self.gamma)
         for s in self.streams:
             cur.wait_stream(s)
 
@@ -193,12 +202,15 @@ def ssq_cwt_batch(xs, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t
             _runners.popitem(last=False)
     else:
         _runners.move_to_end(key)
+    # each result gets its own scales tensor; the copies are queued before
+    # the pass so that they overlap it
+    scs = [runner.result(m)[2] for m in range(len(xs))]
+    scs = [g.scales.clone() if astensor else g.g['scales'].copy() for g in scs]
     runner.run([_as_signal(x, device) for x in xs])
     out = []
     for m in range(len(xs)):
         Tx, Wx, grp = runner.result(m)
         if copy:
             Tx, Wx = Tx.clone(), Wx.clone()
-        sc = grp.scales.clone() if astensor else grp.g['scales'].copy()
-        out.append((to_output(Tx, astensor), to_output(Wx, astensor), grp.freqs, sc))
+        out.append((to_output(Tx, astensor), to_output(Wx, astensor), grp.freqs, scs[m]))
     return out
