@@ -1846,7 +1846,11 @@ __global__ __launch_bounds__(256) void k_adj_ybar(const float2* __restrict__ gTx
                                                   int64_t N, float2* __restrict__ yb) {
   extern __shared__ float2 s_tx[];           // [nf][W]
   const int b = blockIdx.y;
-  const int64_t j0 = (int64_t)blockIdx.x * W;
+  // XCD-aware tile order: neighbouring tiles, which share lines of bins,
+  // meet in one L2
+  const int tile = xcd_task(blockIdx.x, (int)((N + W - 1) / W));
+  if (tile < 0) return;
+  const int64_t j0 = (int64_t)tile * W;
   const float2* tx = gTx + (int64_t)b * nf * N;
   const int tot_t = nf * W;
   for (int e0 = 0; e0 < tot_t; e0 += 256 * kYbU) {
@@ -2401,7 +2405,7 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
       return SSQK_ERR_UNSUPPORTED;
     }
     const size_t lds = (size_t)nf * W * sizeof(float2);
-    dim3 grid((unsigned)((N + W - 1) / W), (unsigned)batch);
+    dim3 grid(xcd_grid((unsigned)((N + W - 1) / W)), (unsigned)batch);
     float2* yb = (float2*)(ws + off[0]);
     static bool attr[3] = {false, false, false};
     auto go = [&](auto kern, int wi) -> int {
