@@ -153,7 +153,10 @@ typedef struct ssqk_signal {
  * of the plan is launched once for all `batch` signals.  sig is a host array
  * [batch]; each signal needs N <= the plan's N and n1 + N <= L (its own
  * p2up padding, utils/common.py:32-51).  Outputs as ssqk_cwt_exec without
- * dWx and w.  workspace: ssqk_cwt_ragged_workspace_size bytes.  Replaces a
+ * dWx and w.  workspace: ssqk_cwt_ragged_workspace_size bytes; the table is
+ * uploaded into it, and sig = NULL reuses the table a previous call left in
+ * the same workspace (for graph capture: no host copy in the pass; the table
+ * is then not checked again).  Replaces a
  * loop of cwt() calls over signals of different lengths (the reference
  * batches only equal lengths, _cwt.py:167-177); no reference counterpart. */
 int ssqk_cwt_ragged_workspace_size(ssqk_cwt_plan plan, int64_t batch, size_t* bytes);
@@ -172,11 +175,12 @@ int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* cst,
                  void* stream);
 
 /* ssqk_squeeze over a ragged batch (host array sig [batch], offsets as for
- * ssqk_cwt_exec_ragged), in one launch; workspace: batch * 64 bytes of
- * device memory.  No reference counterpart (see ssqk_cwt_exec_ragged). */
+ * ssqk_cwt_exec_ragged; nmax >= every N), in one launch; workspace: batch *
+ * 64 bytes of device memory that receives the table (sig = NULL reuses it).
+ * No reference counterpart (see ssqk_cwt_exec_ragged). */
 int ssqk_squeeze_ragged(const void* Wx, const uint16_t* bins, const float* cst,
                         const ssqk_signal* sig, int64_t batch, int64_t na, int64_t nf,
-                        void* Tx, void* workspace, void* stream);
+                        int64_t nmax, void* Tx, void* workspace, void* stream);
 
 /* bins from (Wx, dWx) with the fused phase transform; Sfs (float32 [na])
  * non-NULL selects the STFT form |Sfs[i] - Im(dWx/Wx)/2pi| (algos.py:956-984).

@@ -16,9 +16,13 @@ the runner's input buffer and queues the pass again.  Outputs are views of
 packed buffers that the next pass overwrites unless ``copy=True``.  Groups
 are spread over the streams largest first, each to the least loaded one.
 
-``graph=True`` (capturing the pass into a HIP graph) is switched off: its
-first run ended the process with a segmentation fault inside the runtime's
-end-of-capture call, cause not yet found, so it raises NotImplementedError.
+``graph=True`` (capturing the pass into a HIP graph after one eager pass)
+is switched off and raises NotImplementedError: the process ended with a
+segmentation fault inside torch's end-of-capture call, first with one plan
+per signal and again with the grouped launches after every allocation and
+host copy had been taken out of the captured pass (plans build their tables
+up front; the captured pass reuses the descriptor tables left in the
+workspaces).  The cause is not found.
 """
 from collections import OrderedDict
 
@@ -55,6 +59,7 @@ class _Group:
         self.g, self.na = g, g['na']
         Ns = [lengths[m] for m in members]
         Nmax = max(Ns)
+        self.nmax = Nmax
         L, n1max, pad = _padding(Nmax, padtype)
         self.plan = _lib.cwt_plan(device, Nmax, L, n1max, pad, wav.kind,
                                   _wavelet_kernel_params(wav), g['scales64'], g['dt'])
@@ -85,11 +90,16 @@ class _Group:
         return (self.Tx[na * off:na * (off + N)].view(na, N),
                 self.Wx[na * off:na * (off + N)].view(na, N))
 
-    def launch(self, gamma):
-        self.plan.exec_ragged(self.x, self.sig, self.Wx, self.bins, self.g['bp'], gamma,
-                              True, self.ws)
-        _lib.squeeze_ragged(self.Wx, self.bins, self.cst, self.sig, self.na, self.na,
-                            self.Tx, self.sq_ws)
+    def launch(self, gamma, tables_uploaded=False):
+        """Queue the group's transform and squeeze; tables_uploaded: the
+        descriptor tables are already in the workspaces (no host copy, as
+        inside a graph capture)."""
+        sig = None if tables_uploaded else self.sig
+        n = len(self.members)
+        self.plan.exec_ragged(self.x, sig, self.Wx, self.bins, self.g['bp'], gamma,
+                              True, self.ws, batch=n)
+        _lib.squeeze_ragged(self.Wx, self.bins, self.cst, sig, self.na, self.na, self.nmax,
+                            self.Tx, self.sq_ws, batch=n)
 
 
 class _BatchRunner:
@@ -123,6 +133,7 @@ class _BatchRunner:
             for k, m in enumerate(grp.members):
                 self.where[m] = (gi, k)
         self.gamma = gamma
+        self.graph = None
         self.streams = [torch.cuda.Stream(device=device) for _ in range(nstreams)]
         # largest group first, each to the least loaded stream (work ~ na * sum N)
         load = [0.0] * nstreams
@@ -132,7 +143,7 @@ class _BatchRunner:
             load[s] += self.groups[gi].work
             self.order.append((gi, s))
 
-    def _launch(self):
+    def _launch(self, tables_uploaded=False):
         """Queue every group's ragged transform and squeeze; the streams fork
         from and join into the current stream."""
         cur = torch.cuda.current_stream(self.device)
@@ -140,14 +151,27 @@ class _BatchRunner:
             s.wait_stream(cur)
         for gi, si in self.order:
             with torch.cuda.stream(self.streams[si]):
-                self.groups[gi].launch(self.gamma)
+                self.groups[gi].launch(self.gamma, tables_uploaded)
         for s in self.streams:
             cur.wait_stream(s)
 
-    def run(self, xs):
+    def run(self, xs, graph=False):
+        """Pack the signals into the groups' input buffers (outside any graph:
+        the caller's tensors change from call to call), then run the pass
+        eagerly or by replaying its graph.  The graph is captured after one
+        eager pass, so every kernel and table has been used once before."""
         for grp in self.groups:
             torch.cat([xs[m] for m in grp.members], out=grp.x)
-        self._launch()
+        if not graph:
+            self._launch()
+        elif self.graph is None:
+            self._launch()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._launch(tables_uploaded=True)
+            self.graph = g
+        else:
+            self.graph.replay()
 
     def result(self, m):
         gi, k = self.where[m]
@@ -181,7 +205,7 @@ def ssq_cwt_batch(xs, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t
         raise RuntimeError("ssqueezepy_amd needs a ROCm GPU")
     if graph:
         raise NotImplementedError("graph capture of the batched pass is switched off "
-                                  "(end of capture crashed; cause not found yet)")
+                                  "(the end of capture crashed; cause not found)")
     xs = list(xs)
     if any(getattr(x, 'ndim', 1) != 1 for x in xs):
         raise ValueError("ssq_cwt_batch takes a sequence of 1D signals")
@@ -206,7 +230,7 @@ def ssq_cwt_batch(xs, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t
     # the pass so that they overlap it
     scs = [runner.result(m)[2] for m in range(len(xs))]
     scs = [g.scales.clone() if astensor else g.g['scales'].copy() for g in scs]
-    runner.run([_as_signal(x, device) for x in xs])
+    runner.run([_as_signal(x, device) for x in xs], graph=bool(graph))
     out = []
     for m in range(len(xs)):
         Tx, Wx, grp = runner.result(m)

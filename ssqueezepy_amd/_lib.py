@@ -90,8 +90,8 @@ _SIGS = {
                                      c_void_p, c_void_p, ctypes.POINTER(BinParams),
                                      c_double, c_int, c_void_p, c_void_p]),
     'ssqk_squeeze_ragged': (c_int, [c_void_p, c_void_p, c_void_p, ctypes.POINTER(Signal),
-                                    c_int64, c_int64, c_int64, c_void_p, c_void_p,
-                                    c_void_p]),
+                                    c_int64, c_int64, c_int64, c_int64, c_void_p,
+                                    c_void_p, c_void_p]),
     'ssqk_cwt_adjoint': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     'ssqk_cwt_adjoint_workspace_size': (c_int, [c_void_p, c_int64, c_int,
@@ -227,11 +227,13 @@ class CwtPlan:
         check(load().ssqk_cwt_ragged_workspace_size(self._h, int(batch), ctypes.byref(n)))
         return n.value
 
-    def exec_ragged(self, x, sig, Wx, bins, bp, gamma, strict, workspace):
+    def exec_ragged(self, x, sig, Wx, bins, bp, gamma, strict, workspace, batch=None):
         """Forward CWT + squeeze rows of the signals of ``sig`` (a
-        signal_table) packed in the flat device buffers x, Wx, bins."""
+        signal_table) packed in the flat device buffers x, Wx, bins; sig=None
+        (with ``batch``) reuses the table already in ``workspace``."""
         check(load().ssqk_cwt_exec_ragged(
-            self._h, _ptr(x), sig, len(sig), _ptr(Wx), _ptr(bins),
+            self._h, _ptr(x), sig, len(sig) if sig is not None else int(batch),
+            _ptr(Wx), _ptr(bins),
             ctypes.byref(bp) if bp is not None else None, float(gamma),
             int(bool(strict)), _ptr(workspace), _stream(x.device)))
 
@@ -296,11 +298,14 @@ def cwt_plan(device, N, L, n1, pad, kind, params, scales64, dt):
 
 
 # ---------------------------------------------------------- thin wrappers
-def squeeze_ragged(Wx, bins, cst, sig, na, nf, Tx, workspace):
+def squeeze_ragged(Wx, bins, cst, sig, na, nf, nmax, Tx, workspace, batch=None):
     """ssqk_squeeze over the signals of ``sig`` packed in flat buffers;
-    workspace: >= 64 * len(sig) bytes of device memory."""
-    check(load().ssqk_squeeze_ragged(_ptr(Wx), _ptr(bins), _ptr(cst), sig, len(sig), int(na),
-                                     int(nf), _ptr(Tx), _ptr(workspace), _stream(Wx.device)))
+    workspace: >= 64 * batch bytes of device memory (sig=None reuses the
+    table already there)."""
+    n = len(sig) if sig is not None else int(batch)
+    check(load().ssqk_squeeze_ragged(_ptr(Wx), _ptr(bins), _ptr(cst), sig, n, int(na),
+                                     int(nf), int(nmax), _ptr(Tx), _ptr(workspace),
+                                     _stream(Wx.device)))
 
 def squeeze(Wx, bins, cst, nf, Tx):
     batch = 1 if Wx.ndim == 2 else Wx.shape[0]

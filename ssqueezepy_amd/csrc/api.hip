@@ -451,22 +451,29 @@ extern "C" int ssqk_squeeze(const void* Wx, const uint16_t* bins, const float* c
 
 extern "C" int ssqk_squeeze_ragged(const void* Wx, const uint16_t* bins, const float* cst,
                                    const ssqk_signal* sig, int64_t batch, int64_t na,
-                                   int64_t nf, void* Tx, void* workspace, void* stream) {
-  if (!Wx || !bins || !cst || !Tx || !sig || !workspace || batch < 1 || batch > 65535 ||
-      na < 1 || nf < 1 || nf >= kNoBin) {
+                                   int64_t nf, int64_t nmax, void* Tx, void* workspace,
+                                   void* stream) {
+  if (!Wx || !bins || !cst || !Tx || !workspace || batch < 1 || batch > 65535 ||
+      na < 1 || nf < 1 || nf >= kNoBin || nmax < 1 || nmax * kSqGS >= ((int64_t)1 << 31)) {
     ssqk_host::set_error("bad arguments to ssqk_squeeze_ragged");
     return SSQK_ERR_ARG;
   }
+  if (!sig)                                  // the table this workspace holds
+    return squeeze_impl(Wx, bins, cst, batch, na, nf, nmax, Tx, (const SigDev*)workspace,
+                        stream);
   std::vector<SigDev> d((size_t)batch);
-  int64_t nmax = 0;
   for (int64_t b = 0; b < batch; ++b) {
     const ssqk_signal& g = sig[b];
     if (g.N < 1 || g.N * kSqGS >= ((int64_t)1 << 31) || g.out_off < 0 || g.tx_off < 0) {
       ssqk_host::set_error("signal %lld: bad length or offsets", (long long)b);
       return SSQK_ERR_ARG;
     }
+    if (g.N > nmax) {
+      ssqk_host::set_error("signal %lld: N=%lld above nmax=%lld", (long long)b,
+                           (long long)g.N, (long long)nmax);
+      return SSQK_ERR_ARG;
+    }
     d[b] = SigDev{(int)g.N, (int)g.n1, g.x_off, g.out_off, g.tx_off};
-    nmax = std::max<int64_t>(nmax, g.N);
   }
   SSQK_CHECK(hipMemcpyAsync(workspace, d.data(), d.size() * sizeof(SigDev),
                             hipMemcpyHostToDevice, (hipStream_t)stream));

@@ -1434,6 +1434,24 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       return rc;
     }
   }
+  // every twiddle and phase table the plan's launches use, created now: the
+  // lazy creation does a synchronous copy, which is not allowed while the
+  // caller's stream is being captured into a graph
+  {
+    std::vector<int> sizes;
+    for (const auto& c : p->classes) sizes.push_back(c.M);
+    for (const auto& o : p->os) sizes.push_back(o.F);
+    if (!p->full.empty()) { sizes.push_back(p->L1); sizes.push_back(p->L2); }
+    if (p->L <= kSmallFwd) sizes.push_back((int)p->L);
+    else { sizes.push_back(p->fL1); sizes.push_back(p->fL2); }
+    bool ok = ssqk_host::phase_table((int)p->L) != nullptr;
+    for (int m : sizes) ok = ok && (m < 2 || ssqk_host::twiddles(m) != nullptr);
+    if (!ok) {
+      ssqk_cwt_plan_destroy(p);
+      ssqk_host::set_error("could not create the plan's twiddle tables");
+      return SSQK_ERR_HIP;
+    }
+  }
   *out = p;
   return SSQK_OK;
 }
@@ -1771,7 +1789,7 @@ extern "C" int ssqk_cwt_exec_ragged(ssqk_cwt_plan p, const float* x, const ssqk_
                                     int64_t batch, void* Wx, uint16_t* bins,
                                     const ssqk_bin_params* bp, double gamma, int gamma_strict,
                                     void* workspace, void* stream) {
-  if (!p || !x || !sig || !Wx || !workspace || batch < 1 || batch > 65535) {
+  if (!p || !x || !Wx || !workspace || batch < 1 || batch > 65535) {
     ssqk_host::set_error("bad arguments to ssqk_cwt_exec_ragged");
     return SSQK_ERR_ARG;
   }
@@ -1783,6 +1801,11 @@ extern "C" int ssqk_cwt_exec_ragged(ssqk_cwt_plan p, const float* x, const ssqk_
     ssqk_host::set_error("omax=%lld out of range for uint16 bins", (long long)bp->omax);
     return SSQK_ERR_ARG;
   }
+  char* ws = (char*)workspace;
+  SigDev* dsig = (SigDev*)(ws + ragged_sig_offset(p, batch));
+  if (!sig)                                  // the table this workspace holds
+    return exec_impl(p, x, batch, Wx, nullptr, nullptr, bins, bp, gamma, gamma_strict,
+                     workspace, (hipStream_t)stream, dsig);
   // every signal must fit the plan: the overlap-save blocks cover the
   // plan's N outputs and the padded length is the plan's L
   std::vector<SigDev> d((size_t)batch);
@@ -1797,8 +1820,6 @@ extern "C" int ssqk_cwt_exec_ragged(ssqk_cwt_plan p, const float* x, const ssqk_
     }
     d[b] = SigDev{(int)g.N, (int)g.n1, g.x_off, g.out_off, g.tx_off};
   }
-  char* ws = (char*)workspace;
-  SigDev* dsig = (SigDev*)(ws + ragged_sig_offset(p, batch));
   SSQK_CHECK(hipMemcpyAsync(dsig, d.data(), d.size() * sizeof(SigDev), hipMemcpyHostToDevice,
                             (hipStream_t)stream));
   return exec_impl(p, x, batch, Wx, nullptr, nullptr, bins, bp, gamma, gamma_strict,

@@ -19,9 +19,7 @@ def _rel(a, b):
     return (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item()
 
 
-@pytest.mark.parametrize('graph', [False, pytest.param(True, marks=pytest.mark.skip(
-    reason="graph capture of the batched pass is switched off: its first run crashed "
-           "(segmentation fault at the end of capture), cause not found"))])
+@pytest.mark.parametrize('graph', [False])
 def test_batch_equals_per_signal(graph):
     """Groups by padded length (8192: 4096, 5000, 4500; 16384: 7777; 32768:
     12000, 16384; 65536: 30001): the longest of each group runs the plan its
@@ -58,6 +56,60 @@ def test_batch_equals_per_signal(graph):
                 # the neighbouring row: the column sums do not see that
                 assert _rel(Tx.sum(0), Tx_r.sum(0)) <= 1e-5
                 assert _rel(Tx, Tx_r) <= 1e-4
+
+
+@pytest.mark.skip(reason="graph capture of the batched pass is switched off: the end of "
+                         "capture crashed (segmentation fault in torch's capture_end) twice, "
+                         "the second time with no allocation or host copy in the captured "
+                         "pass; cause not found")
+def test_graph_replays_equal_the_eager_pass():
+    """graph=True: eager pass + capture on the first call, replays after it;
+    every replay (new data, eager work and synchronizations in between)
+    equals the eager pass on the same data bit for bit."""
+    import torch
+    import ssqueezepy_amd as sq
+    wav = sq.Wavelet(('morlet', {'mu': 6.}))
+    lengths = [4096, 5000, 7777, 12000, 4500, 30001]
+    for seed in range(4):
+        xs = [chirp_tone_noise(n, 300 + 10 * seed + k) for k, n in enumerate(lengths)]
+        print(f"graph pass seed={seed}", flush=True)
+        out_g = sq.ssq_cwt_batch(xs, wav, graph=True, copy=True)
+        torch.cuda.synchronize()
+        out_e = sq.ssq_cwt_batch(xs, wav, graph=False, copy=True)
+        torch.cuda.synchronize()
+        for (Tg, Wg, fg, sg), (Te, We, fe, se) in zip(out_g, out_e):
+            assert torch.equal(Wg, We)
+            assert torch.equal(Tg, Te)
+            assert np.array_equal(fg, fe) and torch.equal(sg, se)
+
+
+def test_graph_mode_is_switched_off():
+    import ssqueezepy_amd as sq
+    with pytest.raises(NotImplementedError, match="switched off"):
+        sq.ssq_cwt_batch([chirp_tone_noise(5000, 0)], 'gmw', graph=True)
+
+
+def test_ragged_exec_reuses_the_uploaded_table():
+    """sig=None reuses the descriptor table a previous call left in the
+    workspace: same outputs as the call that uploaded it."""
+    import torch
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd import _batch
+    wav = sq.Wavelet('gmw')
+    lengths = [6100, 7300, 5900]
+    xs = [chirp_tone_noise(n, 70 + k) for k, n in enumerate(lengths)]
+    out = sq.ssq_cwt_batch(xs, wav, copy=True)
+    torch.cuda.synchronize()
+    runner = next(r for k, r in _batch._runners.items() if k[1] == tuple(lengths))
+    for grp in runner.groups:
+        grp.Wx.zero_()
+        grp.Tx.zero_()
+        grp.launch(runner.gamma, tables_uploaded=True)
+    torch.cuda.synchronize()
+    for m, (Tx, Wx, _, _) in enumerate(out):
+        Tx2, Wx2, _ = runner.result(m)
+        assert torch.equal(Wx, Wx2)
+        assert torch.equal(Tx, Tx2)
 
 
 def test_ragged_batch_of_equal_lengths_is_bitwise():
