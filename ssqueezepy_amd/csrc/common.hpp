@@ -67,28 +67,20 @@ __device__ __forceinline__ int bin_exact(double w, const BinDev& p) {
 // when w lies within 1e-3 bins of a rounding edge (or near the piecewise
 // transition, or out of range): the float64 evaluation then decides.
 __device__ __forceinline__ int bin_fast_w(float w, const BinDev& p) {
-  float u;
-  bool near = false;
-  const float wl = p.type == 0 ? 0.0f : __log2f(w);
-  if (p.type == 0) {
-    u = (w - p.v0f) * p.inv_d0f;
-  } else if (p.type == 2) {
-    const float dd = wl - p.v1f;
-    near = fabsf(dd) < 1e-4f * (1.0f + fabsf(p.v1f));
-    u = dd > 0.0f ? dd * p.inv_d1f : (wl - p.v0f) * p.inv_d0f;
-  } else {
-    u = (wl - p.v0f) * p.inv_d0f;
-  }
+  // the three grid types in one branch-free form: linear bins use w itself,
+  // log bins log2 w; piecewise bins past the transition use the second grid
+  // and start at idx1 (only those are clamped to omax from above there)
+  const bool lin = p.type == 0, pw = p.type == 2;
+  const float x = lin ? w : __log2f(w);
+  const float dd = x - p.v1f;
+  const bool hi = pw && dd > 0.0f;
+  const float u = hi ? dd * p.inv_d1f : (x - p.v0f) * p.inv_d0f;
   const float fr = u - floorf(u);
-  near = near || fabsf(fr - 0.5f) < 1e-3f || !(fabsf(u) < 1e6f);
+  const bool near = (pw && fabsf(dd) < 1e-4f * (1.0f + fabsf(p.v1f))) ||
+                    fabsf(fr - 0.5f) < 1e-3f || !(fabsf(u) < 1e6f);
   if (near) return -1;
-  float k;
-  if (p.type == 0 || p.type == 1) {
-    k = fminf(rintf(fmaxf(u, 0.0f)), (float)p.omax);
-  } else {
-    if (wl > p.v1f) k = fminf(rintf(u) + (float)p.idx1, (float)p.omax);
-    else k = fmaxf(rintf(u), 0.0f);
-  }
+  const float top = (pw && !hi) ? 3.0e38f : (float)p.omax;
+  const float k = fminf(fmaxf(rintf(u) + (hi ? (float)p.idx1 : 0.0f), 0.0f), top);
   const int ki = (int)k;
   return p.flip ? p.omax - ki : ki;
 }

@@ -26,6 +26,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace ssqk;
@@ -66,6 +67,7 @@ struct CwtArgs {
   const float* psih;        // psih(s_i xi_k) table, scale i at psih_off[i] + (k - ka_i)
   const int64_t* psih_off;
   const SigDev* sig;        // ragged batch: per-signal N, n1, offsets (else null)
+  int plain;                // outputs are Wx and strict bins only
 };
 
 // Length, left pad and offsets of signal b: from the ragged-batch table, or
@@ -126,23 +128,43 @@ __device__ __forceinline__ int64_t pad_src(int64_t n, int64_t N, int64_t n1, int
   }
 }
 
+// Output row of scale i of one signal: element offset of padded position 0
+// (the row's outputs are the padded positions [n1, n1 + N)).  Uniform per
+// workgroup, so the per-point address is one 32-bit index.
+struct OutRow {
+  int64_t base;
+  int n1, N;
+};
+__device__ __forceinline__ OutRow out_row(const Sig& s, int i) {
+  return OutRow{s.out_off + (int64_t)i * s.N - s.n1, s.n1, s.N};
+}
+
 // Shared output stage: z, zd are Wx and dWx at padded position n.  Returns
 // false when the point's bin lies at a float32 rounding edge: the caller
 // then finishes the point with emit_edge (the float64 evaluation, long code
 // kept out of unrolled loops).
-__device__ __forceinline__ bool emit_fast(const CwtArgs& a, const Sig& s, int i, int64_t n,
+// PLAIN: the plain ssq_cwt outputs (Wx and strict bins only), with the
+// optional outputs compiled out (CwtArgs::plain selects it at run time).
+template <bool PLAIN>
+__device__ __forceinline__ bool emit_fast(const CwtArgs& a, const OutRow& r, int n,
                                           float2 z, float2 zd) {
-  const int64_t j = n - s.n1;
-  if (j < 0 || j >= s.N) return true;
-  const int64_t o = s.out_off + (int64_t)i * s.N + j;
-  a.Wx[o] = z;
-  if (a.dWx) a.dWx[o] = zd;
+  if ((unsigned)(n - r.n1) >= (unsigned)r.N) return true;
+  a.Wx[r.base + n] = z;
+  if constexpr (PLAIN) {
+    const float m2 = z.x * z.x + z.y * z.y;
+    int k = kNoBin;
+    if (m2 > a.gamma2) k = bin_fast_w(phase_fast(zd.x, zd.y, z.x, z.y), a.bp);
+    if (k < 0) return false;
+    a.bins[r.base + n] = (uint16_t)k;
+    return true;
+  }
+  if (a.dWx) a.dWx[r.base + n] = zd;
   const float m2 = z.x * z.x + z.y * z.y;
   float wv = 0.f;
   if (a.w || (a.bins && !a.strict)) {
     wv = (m2 < a.gamma2) ? INFINITY
                          : fabsf((zd.y * z.x - zd.x * z.y) / (m2 * kTwoPiF));
-    if (a.w) a.w[o] = wv;
+    if (a.w) a.w[r.base + n] = wv;
   }
   if (a.bins) {
     int k = kNoBin;
@@ -152,14 +174,13 @@ __device__ __forceinline__ bool emit_fast(const CwtArgs& a, const Sig& s, int i,
       k = bin_fast_w(wv, a.bp);
     }
     if (k < 0) return false;
-    a.bins[o] = (uint16_t)k;
+    a.bins[r.base + n] = (uint16_t)k;
   }
   return true;
 }
 
-__device__ __forceinline__ void emit_edge(const CwtArgs& a, const Sig& s, int i, int64_t n,
+__device__ __forceinline__ void emit_edge(const CwtArgs& a, const OutRow& r, int n,
                                           float2 z, float2 zd) {
-  const int64_t o = s.out_off + (int64_t)i * s.N + (n - s.n1);
   int k;
   if (a.strict) {
     k = bin_edge_dwx(zd.x, zd.y, z.x, z.y, a.bp);
@@ -167,12 +188,13 @@ __device__ __forceinline__ void emit_edge(const CwtArgs& a, const Sig& s, int i,
     const float m2 = z.x * z.x + z.y * z.y;
     k = bin_exact((double)fabsf((zd.y * z.x - zd.x * z.y) / (m2 * kTwoPiF)), a.bp);
   }
-  a.bins[o] = (uint16_t)k;
+  a.bins[r.base + n] = (uint16_t)k;
 }
 
-__device__ __forceinline__ void emit(const CwtArgs& a, const Sig& s, int i, int64_t n,
-                                     float2 z, float2 zd) {
-  if (!emit_fast(a, s, i, n, z, zd)) emit_edge(a, s, i, n, z, zd);
+template <bool PLAIN>
+__device__ __forceinline__ void emit(const CwtArgs& a, const OutRow& r, int n, float2 z,
+                                     float2 zd) {
+  if (!emit_fast<PLAIN>(a, r, n, z, zd)) emit_edge(a, r, n, z, zd);
 }
 
 // X = psih(s xi_k) v and Xd = X * i xi_k / dt for a positive bin k, where
@@ -295,24 +317,29 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
   lds_fft<M, ROWS, +1, kNT, 2>(bufs, tw);
 
   static_assert(IR <= 32, "edge mask");
+  const OutRow orow = out_row(sg, i);
   uint32_t edge = 0;
+  auto emit_all = [&](auto plain) {
 #pragma unroll
-  for (int it = 0; it < IR; ++it) {
-    const int e = threadIdx.x + it * kNT;
-    const int r = e % ROWS, q = e / ROWS;
-    const int ph_a = a0 + r;
-    const int64_t n = ph_a + (int64_t)D * q;
-    if (e < ROWS * M && ph_a < D && n >= sg.n1 && n < (int64_t)sg.n1 + sg.N &&
-        !emit_fast(a, sg, i, n, sW[Lds<M>::idx(r, q)], sD[Lds<M>::idx(r, q)]))
-      edge |= 1u << it;
-  }
+    for (int it = 0; it < IR; ++it) {
+      const int e = threadIdx.x + it * kNT;
+      const int r = e % ROWS, q = e / ROWS;
+      const int ph_a = a0 + r;
+      const int n = ph_a + D * q;             // < L <= 2^30
+      if (e < ROWS * M && ph_a < D &&
+          !emit_fast<decltype(plain)::value>(a, orow, n, sW[Lds<M>::idx(r, q)],
+                                             sD[Lds<M>::idx(r, q)]))
+        edge |= 1u << it;
+    }
+  };
+  if (a.plain) emit_all(std::true_type{});
+  else emit_all(std::false_type{});
   while (edge) {
     const int it = __builtin_ctz(edge);
     edge &= edge - 1;
     const int e = threadIdx.x + it * kNT;
     const int r = e % ROWS, q = e / ROWS;
-    emit_edge(a, sg, i, a0 + r + (int64_t)D * q, sW[Lds<M>::idx(r, q)],
-              sD[Lds<M>::idx(r, q)]);
+    emit_edge(a, orow, a0 + r + D * q, sW[Lds<M>::idx(r, q)], sD[Lds<M>::idx(r, q)]);
   }
 }
 
@@ -426,14 +453,29 @@ __global__ __launch_bounds__(kPF / 8) void k_cwt_fullB(CwtArgs a, const int4* __
   float2* const bufs[2] = {sW, sD};
   lds_fft<L1, G, +1, kNT, 2>(bufs, tw);
   const float invL = 1.0f / (float)a.L;
-  const Sig sg = sig_at(a, b);
-  for (int e = threadIdx.x; e < G * L1; e += kNT) {
+  const OutRow orow = out_row(sig_at(a, b), i);
+  static_assert((G * L1 + kNT - 1) / kNT <= 32, "edge mask");
+  uint32_t edge = 0;
+  auto emit_all = [&](auto plain) {
+    for (int it = 0, e = threadIdx.x; e < G * L1; ++it, e += kNT) {
+      const int g = e % G, n1 = e / G;
+      const int n = L2 * n1 + n2_0 + g;
+      if ((unsigned)(n - orow.n1) >= (unsigned)orow.N) continue;
+      float2 z = sW[Lds<L1>::idx(g, n1)], zd = sD[Lds<L1>::idx(g, n1)];
+      z.x *= invL; z.y *= invL; zd.x *= invL; zd.y *= invL;
+      if (!emit_fast<decltype(plain)::value>(a, orow, n, z, zd)) edge |= 1u << it;
+    }
+  };
+  if (a.plain) emit_all(std::true_type{});
+  else emit_all(std::false_type{});
+  while (edge) {                             // points at a bin rounding edge
+    const int it = __builtin_ctz(edge);
+    edge &= edge - 1;
+    const int e = threadIdx.x + it * kNT;
     const int g = e % G, n1 = e / G;
-    const int64_t n = (int64_t)L2 * n1 + n2_0 + g;
-    if (n < sg.n1 || n >= (int64_t)sg.n1 + sg.N) continue;
     float2 z = sW[Lds<L1>::idx(g, n1)], zd = sD[Lds<L1>::idx(g, n1)];
     z.x *= invL; z.y *= invL; zd.x *= invL; zd.y *= invL;
-    emit(a, sg, i, n, z, zd);
+    emit_edge(a, orow, L2 * n1 + n2_0 + g, z, zd);
   }
 }
 
@@ -533,14 +575,31 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
     }
     float2* const bufs[2] = {sA, sD};
     lds_fft<F, SR, +1, NT, 2>(bufs, tw);
-    for (int e = tx; e < SR * g.B; e += NT) {
+    // (SR * B <= SR * F = 8 NT, so a thread has at most 8 points)
+    uint32_t edge = 0;
+    auto emit_all = [&](auto plain) {
+      for (int it = 0, e = tx; e < SR * g.B; ++it, e += NT) {
+        const int r = e / g.B, tt = e - r * g.B;
+        if (s0 + r >= g.ns) continue;
+        const int64_t j = j_base + tt;
+        if (j >= sg.N) continue;
+        const int pos = g.H + tt;
+        if (!emit_fast<decltype(plain)::value>(a, out_row(sg, g.i0 + s0 + r),
+                                               sg.n1 + (int)j, sA[Lds<F>::idx(r, pos)],
+                                               sD[Lds<F>::idx(r, pos)]))
+          edge |= 1u << it;
+      }
+    };
+    if (a.plain) emit_all(std::true_type{});
+    else emit_all(std::false_type{});
+    while (edge) {                           // points at a bin rounding edge
+      const int it = __builtin_ctz(edge);
+      edge &= edge - 1;
+      const int e = tx + it * NT;
       const int r = e / g.B, tt = e - r * g.B;
-      if (s0 + r >= g.ns) continue;
-      const int64_t j = j_base + tt;
-      if (j >= sg.N) continue;
       const int pos = g.H + tt;
-      emit(a, sg, g.i0 + s0 + r, (int64_t)sg.n1 + j, sA[Lds<F>::idx(r, pos)],
-           sD[Lds<F>::idx(r, pos)]);
+      emit_edge(a, out_row(sg, g.i0 + s0 + r), sg.n1 + (int)(j_base + tt),
+                sA[Lds<F>::idx(r, pos)], sD[Lds<F>::idx(r, pos)]);
     }
     __syncthreads();
   }
@@ -1640,6 +1699,7 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
   a.psih = p->d_psih;
   a.psih_off = p->d_psih_off;
   a.sig = sig;
+  a.plain = bins && gamma_strict && !dWx && !w;
   if (!a.ptab) {
     ssqk_host::set_error("could not create the phase table");
     return SSQK_ERR_HIP;
