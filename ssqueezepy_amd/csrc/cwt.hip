@@ -1983,8 +1983,10 @@ __global__ __launch_bounds__(256) void k_adj_ybar(const float2* __restrict__ gTx
 }
 
 // Transposed band transform of scale i over the phases [a0, a0 + ph*ROWS):
-//   Y[ka + m] = sum_a w_L^{-(ka+m) a} FFT_M( u[a + D q] w_M^{-ka q} )[m],
-// u = ybar_i placed at the padded positions [n1, n1 + N).  One partial
+//   Y[ka + m] = sum_a w_L^{-(ka+m) a} FFT_M( u[a + D q] w_M^{-ka q} )[m]
+//             = sum_a w_L^{-(ka+m) a} FFT_M( u[a + D q] )[(ka + m) mod M],
+// u = ybar_i placed at the padded positions [n1, n1 + N): the input twiddle
+// is absorbed by reading the FFT at the cyclic position.  One partial
 // M-vector per task, summed over its phases in a fixed order.
 template <int M>
 __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
@@ -2002,32 +2004,30 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
   const int4 t = tasks[task];
   const int b = blockIdx.y;
   const int i = t.x, a0 = t.y, ka = t.z, lD = t.w;
-  const int64_t D = (int64_t)1 << lD;
+  const int D = 1 << lD;                     // padded positions < L <= 2^30
   const float2* yb = a.yb + (int64_t)b * a.yb_stride + (int64_t)i * a.N;
-  const int L = a.L;
+  const int L = a.L, n1 = (int)a.n1, N = (int)a.N;
   float2 acc[IT];
 #pragma unroll
   for (int it = 0; it < IT; ++it) acc[it] = make_float2(0.f, 0.f);
   for (int p = 0; p < ph; ++p) {
-    const int64_t ab = a0 + (int64_t)p * ROWS;
+    const int ab = a0 + p * ROWS;
     if (ab >= D) break;                      // uniform
     float2 v[IT];
-    float2 w[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = threadIdx.x + it * NT;
       const int r = e / M, q = e - r * M;
-      const int64_t j = ab + r + D * q - a.n1;
-      v[it] = yb[(j >= 0 && j < a.N) ? j : 0];
-      w[it] = cconj(phase_tab((uint32_t)ka * (uint32_t)q * (uint32_t)D, a.ptab, L, a.pS));
+      const int j = ab + r + D * q - n1;
+      v[it] = yb[(unsigned)j < (unsigned)N ? j : 0];
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = threadIdx.x + it * NT;
       const int r = e / M, q = e - r * M;
-      const int64_t j = ab + r + D * q - a.n1;
-      const bool in = j >= 0 && j < a.N && ab + r < D;
-      s[Lds<M>::idx(r, q)] = in ? cmul(v[it], w[it]) : make_float2(0.f, 0.f);
+      const int j = ab + r + D * q - n1;
+      const bool in = (unsigned)j < (unsigned)N && ab + r < D;
+      s[Lds<M>::idx(r, q)] = in ? v[it] : make_float2(0.f, 0.f);
     }
     float2* const bufs[1] = {s};
     lds_fft<M, ROWS, -1, NT, 1>(bufs, tw);
@@ -2037,7 +2037,7 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
       const int r = e / M, m = e - r * M;
       const float2 ph_ = cconj(phase_tab((uint32_t)(ka + m) * (uint32_t)(ab + r), a.ptab, L,
                                          a.pS));
-      const float2 y = cmul(s[Lds<M>::idx(r, m)], ph_);
+      const float2 y = cmul(s[Lds<M>::idx(r, (m + ka) & (M - 1))], ph_);
       acc[it].x += y.x;
       acc[it].y += y.y;
     }
