@@ -2082,7 +2082,20 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a, const int4* __res
   if (m >= M) return;
   const float2* pb = partial + (int64_t)b * part_stride + (int64_t)(uint32_t)r.w;
   float2 v = make_float2(0.f, 0.f);
-  for (int t = 0; t < r.z; ++t) {
+  // loads 8 partials ahead, sums them in task order
+  constexpr int U = 8;
+  int t = 0;
+  for (; t + U <= r.z; t += U) {
+    float2 q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) q[u] = pb[(int64_t)(t + u) * M + m];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      v.x += q[u].x;
+      v.y += q[u].y;
+    }
+  }
+  for (; t < r.z; ++t) {
     const float2 q = pb[(int64_t)t * M + m];
     v.x += q.x;
     v.y += q.y;
@@ -2291,15 +2304,29 @@ __global__ __launch_bounds__(256) void k_adj_accum(AdjArgs a, const int2* __rest
   if (k <= a.L / 2) {
     const float2* yt = a.ytab + (int64_t)b * a.ytab_stride;
     const int2 rg = krange[blockIdx.x];      // scales whose windows meet this block
-    for (int i = rg.x; i <= rg.y; ++i) {
-      const int2 sg = seg[i];
-      const int64_t m = k - sg.x;
-      if (m >= 0 && m < sg.y) {
-        const int64_t o = a.psih_off[i] + m;
-        const float ps = a.psih[o];
-        const float2 y = yt[o];
-        acc.x += ps * y.x;
-        acc.y += ps * y.y;
+    // loads 4 scales ahead (out-of-window ones read element 0: both tables
+    // hold at least one), sums in ascending scale order
+    constexpr int U = 4;
+    for (int i0 = rg.x; i0 <= rg.y; i0 += U) {
+      float ps[U];
+      float2 y[U];
+      bool in[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = min(i0 + u, rg.y);
+        const int2 sg = seg[i];
+        const int64_t m = k - sg.x;
+        in[u] = i0 + u <= rg.y && m >= 0 && m < sg.y;
+        const int64_t o = in[u] ? a.psih_off[i] + m : 0;
+        ps[u] = a.psih[o];
+        y[u] = yt[o];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (in[u]) {
+          acc.x += ps[u] * y[u].x;
+          acc.y += ps[u] * y[u].y;
+        }
       }
     }
   }
