@@ -2205,25 +2205,23 @@ __global__ __launch_bounds__(256) void k_adj_os_add(const int4* __restrict__ grp
                                                     const float* __restrict__ rows,
                                                     int64_t rows_stride,
                                                     float* __restrict__ xp) {
+  // positions, spans and block indices are below L <= 2^30 (32-bit), and
+  // L is a power of two (the wrap is a mask)
   const int b = blockIdx.y;
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= L) return;
+  const int Lm = (int)L - 1;
   const float* rb = rows + (int64_t)b * rows_stride;
   float acc = 0.f;
   for (int gi = 0; gi < ngroups; ++gi) {
     const int4 q = grp[gi];
-    const int64_t F = q.x, B = q.y, H = q.z, nblk = q.w;
-    const int64_t span = (nblk - 1) * B + F;
-    int64_t u = (t - (n1 - H)) % L;
-    if (u < 0) u += L;
-    for (; u < span; u += L) {
-      int64_t bk_hi = u / B;
-      if (bk_hi > nblk - 1) bk_hi = nblk - 1;
-      int64_t bk_lo = u - F + 1 <= 0 ? 0 : (u - F + 1 + B - 1) / B;
-      for (int64_t bk = bk_lo; bk <= bk_hi; ++bk) {
-        const int64_t e = u - bk * B;
-        acc += rb[grp_base[gi] + bk * F + e];
-      }
+    const int F = q.x, B = q.y, H = q.z, nblk = q.w;
+    const int span = (nblk - 1) * B + F;
+    const float* rg = rb + grp_base[gi];
+    for (int u = (t - (int)n1 + H) & Lm; u < span; u += (int)L) {
+      const int bk_hi = min((int)((unsigned)u / (unsigned)B), nblk - 1);
+      const int bk_lo = u - F + 1 <= 0 ? 0 : (int)((unsigned)(u - F + B) / (unsigned)B);
+      for (int bk = bk_lo; bk <= bk_hi; ++bk) acc += rg[(int64_t)bk * F + (u - bk * B)];
     }
   }
   xp[(int64_t)b * L + t] += acc;
