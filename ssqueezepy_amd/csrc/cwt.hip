@@ -2013,18 +2013,19 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
   for (int p = 0; p < ph; ++p) {
     const int ab = a0 + p * ROWS;
     if (ab >= D) break;                      // uniform
+    // consecutive lanes take consecutive phases r (consecutive positions)
     float2 v[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = threadIdx.x + it * NT;
-      const int r = e / M, q = e - r * M;
+      const int r = e % ROWS, q = e / ROWS;
       const int j = ab + r + D * q - n1;
       v[it] = yb[(unsigned)j < (unsigned)N ? j : 0];
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = threadIdx.x + it * NT;
-      const int r = e / M, q = e - r * M;
+      const int r = e % ROWS, q = e / ROWS;
       const int j = ab + r + D * q - n1;
       const bool in = (unsigned)j < (unsigned)N && ab + r < D;
       s[Lds<M>::idx(r, q)] = in ? v[it] : make_float2(0.f, 0.f);
