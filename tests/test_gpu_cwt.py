@@ -206,6 +206,26 @@ def test_c1_parity_and_bins():
     assert no == 0
 
 
+def test_c2_parity_and_bins():
+    """C2: N=160,000 GMW(3,60) chirp+tone+noise, nv=32 (293 scales), default
+    log-piecewise grid: Wx and dWx within 1e-5 relative L2 of the float64
+    oracle, every differing squeeze row an edge or float32-noise case, Tx
+    index-matched within 1e-5."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+    N = 160000
+    x = chirp_tone_noise(N, 0)
+    wav = _wav('gmw')
+    g = ssq_cwt_grid(N, wav)
+    print(f"C2 N={N} na={g['na']}", flush=True)
+    Tx, Wx, f, s, dWx = sq.ssq_cwt(x, wav, get_dWx=True)
+    Tx_o, Wx_o, dWx_o, w_o, k_o = _oracle(x, 'gmw', g)
+    k = product_bins(Wx, dWx, g['bp'])
+    r = check_ssq("C2", Tx.cpu().numpy(), Wx.cpu().numpy(), k, Tx_o, Wx_o, dWx_o, w_o, k_o,
+                  g['const'], O.bin_params(g['ssq_freqs'], True))
+    assert r['mismatch'] <= 1e-4 * r['n']
+
+
 def test_matches_reference_golden(golden):
     """ssq_cwt (float32 on the GPU) against the reference's float64 outputs."""
     import ssqueezepy_amd as sq

@@ -32,7 +32,7 @@ def test_stft_vs_oracle(N, n_fft, hop):
     assert rel_l2(dSx.cpu().numpy(), dSx_o) < 1e-5
 
 
-@pytest.mark.parametrize('N,n_fft', [(100, 64), (5000, 256), (30000, 512)])
+@pytest.mark.parametrize('N,n_fft', [(100, 64), (5000, 256), (30000, 512), (160000, 512)])
 def test_ssq_stft_vs_oracle(N, n_fft):
     import torch
     import ssqueezepy_amd as sq
