@@ -135,6 +135,30 @@ def fwd_bwd_batch(torch, sq, B=32, N=16384, steps=10):
             "note": "ms_fwd_bwd includes the loss and its torch backward"}
 
 
+def ssq_stft_line(torch, sq, N=160000, n_fft=512, steps=20):
+    """ssq_stft of the same chirp + tone + noise signal, N=160,000, hann,
+    n_fft=512, hop 1 (Sx and Tx 257 x 160,000 complex64), x already on the
+    device: whole-call time and its HBM roofline (4 B/sample of x read, Sx
+    and Tx written once: 16 B per bin*sample)."""
+    dev = torch.device('cuda', 0)
+    x = torch.as_tensor(chirp_tone_noise(N, 0), device=dev)
+    for _ in range(3):
+        sq.ssq_stft(x, 'hann', n_fft=n_fft)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sq.ssq_stft(x, 'hann', n_fft=n_fft)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    nb = n_fft // 2 + 1
+    alg = 4 * N + 16 * nb * N
+    return {"workload": "ssq_stft, N=%d, hann, n_fft=%d, hop 1 (%d bins)" % (N, n_fft, nb),
+            "ms_per_call": round(ms, 4),
+            "value": round(nb * N / (ms * 1e-3) / 1e9, 3), "unit": "Gbin*samples/s",
+            "alg_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+            "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def _traffic():
     """HBM bytes per k_squeeze launch measured with rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy), kept in
@@ -336,6 +360,7 @@ def main():
     if fb is not None:
         res["fwd_bwd"] = fb
         res["fwd_bwd_batch"] = fwd_bwd_batch(torch, sq)
+    res["ssq_stft"] = ssq_stft_line(torch, sq)
     if not args.no_batched:
         res["batched_uneven"] = batched_uneven(torch, sq)
     if args.autotune:

@@ -141,9 +141,7 @@ __global__ void k_bins_dwx(const float2* __restrict__ Wx, const float2* __restri
     if (m2 > gamma2) {
       if (Sfs) {
         // |Sfs[i] - Im(dWx/Wx)/2pi| (algos.py:971-984), linear bins
-        const float sf = Sfs[o / N];
-        const float wv = fabsf(sf - (zd.y * z.x - zd.x * z.y) / (m2 * kTwoPiF));
-        k = bin_from_w(wv, bp);
+        k = bin_from_sfs(Sfs[o / N], zd.x, zd.y, z.x, z.y, bp);
       } else {
         k = bin_from_dwx(zd.x, zd.y, z.x, z.y, bp);
       }

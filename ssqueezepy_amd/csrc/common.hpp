@@ -108,6 +108,25 @@ __device__ __forceinline__ int bin_from_dwx(float A, float B, float C, float D,
   return k >= 0 ? k : bin_edge_dwx(A, B, C, D, p);
 }
 
+// STFT form |sf - Im(dSx / Sx)| / 2pi (algos.py:956-984) of dSx = A+iB,
+// Sx = C+iD: float32 with the hardware reciprocal, and the float64
+// evaluation of the same inputs for the points bin_fast_w leaves open.
+__device__ __forceinline__ float phase_fast_sfs(float sf, float A, float B, float C,
+                                                float D) {
+  const float m2 = C * C + D * D;
+  return fabsf(sf - (B * C - A * D) * __builtin_amdgcn_rcpf(m2 * kTwoPiF));
+}
+__device__ __forceinline__ int bin_edge_sfs(float sf, float A, float B, float C, float D,
+                                            const BinDev& p) {
+  const double m2 = (double)C * C + (double)D * D;
+  return bin_exact(fabs((double)sf - ((double)B * C - (double)A * D) / (m2 * kTwoPi)), p);
+}
+__device__ __forceinline__ int bin_from_sfs(float sf, float A, float B, float C, float D,
+                                            const BinDev& p) {
+  const int k = bin_fast_w(phase_fast_sfs(sf, A, B, C, D), p);
+  return k >= 0 ? k : bin_edge_sfs(sf, A, B, C, D, p);
+}
+
 // Same for an already computed float32 w (indexed_sum_onfly path,
 // algos.py:153-250): caller has checked that w is finite.
 __device__ __forceinline__ int bin_from_w(float w, const BinDev& p) {

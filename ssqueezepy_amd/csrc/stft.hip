@@ -53,14 +53,22 @@ __global__ __launch_bounds__(kNT) void k_stft(StftArgs a, const float2* __restri
   constexpr int NR = NF / 2 + 1;      // output rows
   __shared__ float2 s[Lds<NF>::elems(F)];
   const int b = blockIdx.y;
-  const int64_t f0 = (int64_t)blockIdx.x * F;
+  const int f0 = blockIdx.x * F;
+  const int nfr = (int)a.n_frames, hop = (int)a.hop, N = (int)a.N, n1 = (int)a.n1;
   const float* x = a.x + (int64_t)b * a.N;
+  // frames wholly inside the signal skip the padding map (positions and
+  // lengths are below 2^31: checked on the host)
+  const bool interior = f0 + F <= nfr && f0 * hop - n1 >= 0 &&
+                        (f0 + F - 1) * hop + NF - 1 - n1 < N;
   for (int e = threadIdx.x; e < F * NF; e += kNT) {
     const int g = e / NF, p = e - g * NF;
-    const int64_t f = f0 + g;
+    const int f = f0 + g;
     float2 z = make_float2(0.f, 0.f);
-    if (f < a.n_frames) {
-      const int64_t src = pad_src_s(f * a.hop + p, a.N, a.n1, a.pad);
+    if (interior) {
+      const float v = x[f * hop + p - n1];
+      z = make_float2(v * a.win[p], a.dwin ? v * a.fs * a.dwin[p] : 0.f);
+    } else if (f < nfr) {
+      const int64_t src = pad_src_s((int64_t)f * hop + p, a.N, a.n1, a.pad);
       const float v = src >= 0 ? x[src] : 0.f;
       z = make_float2(v * a.win[p], a.dwin ? v * a.fs * a.dwin[p] : 0.f);
     }
@@ -69,33 +77,49 @@ __global__ __launch_bounds__(kNT) void k_stft(StftArgs a, const float2* __restri
   float2* const bufs[1] = {s};
   lds_fft<NF, F, -1, kNT, 1>(bufs, tw);
 
-  const int64_t rs = a.n_frames;                     // row stride of outputs
-  const int64_t ob = (int64_t)b * NR * rs;
-  for (int e = threadIdx.x; e < F * NR; e += kNT) {
-    const int g = e % F, k = e / F;
-    const int64_t f = f0 + g;
-    if (f >= a.n_frames) continue;
+  // Sx / dSx rows of this signal; row k of frame f at k * n_frames + f
+  float2* const Sx = a.Sx + (int64_t)b * NR * a.n_frames;
+  float2* const dSx = a.dSx ? a.dSx + (int64_t)b * NR * a.n_frames : nullptr;
+  uint16_t* const bins = a.bins ? a.bins + (int64_t)b * NR * a.n_frames : nullptr;
+  auto spectra = [&](int g, int k, float2& S, float2& dS) {
     const float2 Zk = s[Lds<NF>::idx(g, k)];
     const float2 Zm = s[Lds<NF>::idx(g, (NF - k) & (NF - 1))];
-    float2 S = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
-    float2 dS = make_float2(0.5f * (Zk.y + Zm.y), -0.5f * (Zk.x - Zm.x));
-    if (a.modulated) {
-      const float2 ph = unit_phase((uint64_t)k * (uint64_t)a.s21, NF);
-      S = cmul(S, ph);
-      dS = cmul(dS, ph);
+    S = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+    dS = make_float2(0.5f * (Zk.y + Zm.y), -0.5f * (Zk.x - Zm.x));
+    if (a.modulated && (k & 1)) {
+      // frame ifftshift by s21 = NF / 2 (NF a power of two): the phase
+      // exp(2 pi i k s21 / NF) = (-1)^k
+      S = make_float2(-S.x, -S.y);
+      dS = make_float2(-dS.x, -dS.y);
     }
-    const int64_t o = ob + (int64_t)k * rs + f;
-    a.Sx[o] = S;
-    if (a.dSx) a.dSx[o] = dS;
-    if (a.bins) {
+  };
+  static_assert((F * NR + kNT - 1) / kNT <= 32, "edge mask");
+  uint32_t edge = 0;
+  for (int it = 0, e = threadIdx.x; e < F * NR; ++it, e += kNT) {
+    const int g = e % F, k = e / F;
+    const int f = f0 + g;
+    if (f >= nfr) continue;
+    float2 S, dS;
+    spectra(g, k, S, dS);
+    const int o = k * nfr + f;
+    Sx[o] = S;
+    if (dSx) dSx[o] = dS;
+    if (bins) {
       const float m2 = S.x * S.x + S.y * S.y;
       int kb = kNoBin;
-      if (m2 > a.gamma2) {
-        const float wv = fabsf(a.Sfs[k] - (dS.y * S.x - dS.x * S.y) / (m2 * kTwoPiF));
-        kb = bin_from_w(wv, a.bp);
-      }
-      a.bins[o] = (uint16_t)kb;
+      if (m2 > a.gamma2) kb = bin_fast_w(phase_fast_sfs(a.Sfs[k], dS.x, dS.y, S.x, S.y), a.bp);
+      if (kb < 0) edge |= 1u << it;
+      else bins[o] = (uint16_t)kb;
     }
+  }
+  while (edge) {                             // points at a bin rounding edge:
+    const int it = __builtin_ctz(edge);      // float64 from the same inputs
+    edge &= edge - 1;
+    const int e = threadIdx.x + it * kNT;
+    const int g = e % F, k = e / F;
+    float2 S, dS;
+    spectra(g, k, S, dS);
+    bins[k * nfr + f0 + g] = (uint16_t)bin_edge_sfs(a.Sfs[k], dS.x, dS.y, S.x, S.y, a.bp);
   }
 }
 
@@ -120,6 +144,12 @@ extern "C" int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t 
     ssqk_host::set_error("stft bins need diff_window, Sfs and bin parameters");
     return SSQK_ERR_ARG;
   }
+  // 32-bit positions and output offsets in the kernel
+  const int64_t n_frames = (N + n_fft - 1 - n_fft) / hop + 1;
+  if (N + n_fft >= ((int64_t)1 << 31) || (n_fft / 2 + 1) * n_frames >= ((int64_t)1 << 31)) {
+    ssqk_host::set_error("stft: N=%lld too long for one signal", (long long)N);
+    return SSQK_ERR_ARG;
+  }
   StftArgs a;
   a.x = x;
   a.N = N;
@@ -128,7 +158,7 @@ extern "C" int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t 
   a.n1 = (tot % 2 == 0) ? tot / 2 : tot / 2 + 1;
   a.pad = pad_type;
   a.modulated = modulated;
-  a.s21 = (n_fft % 2 == 1) ? (int)((n_fft + 1) / 2 - 1) : (int)(n_fft / 2);
+  a.s21 = (int)(n_fft / 2);            // n_fft is even (a power of two)
   a.hop = hop;
   a.n_frames = (N + n_fft - 1 - n_fft) / hop + 1;
   a.win = window;
