@@ -2031,7 +2031,8 @@ __global__ __launch_bounds__(band_threads(M)) void k_adj_band(AdjArgs a,
       s[Lds<M>::idx(r, q)] = in ? v[it] : make_float2(0.f, 0.f);
     }
     float2* const bufs[1] = {s};
-    lds_fft<M, ROWS, -1, NT, 1>(bufs, tw);
+    // (at M = 8192, 1024 threads, a radix-32 pass would spill)
+    lds_fft<M, ROWS, -1, NT, 1, (M < 8192)>(bufs, tw);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = threadIdx.x + it * NT;

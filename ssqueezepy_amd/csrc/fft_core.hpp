@@ -154,6 +154,10 @@ template <int DIR>
 struct RegDFT<16, DIR> {
   __device__ __forceinline__ static void run(float2* v) { RegDFTComposite<4, 4, DIR>::run(v); }
 };
+template <int DIR>
+struct RegDFT<32, DIR> {
+  __device__ __forceinline__ static void run(float2* v) { RegDFTComposite<4, 8, DIR>::run(v); }
+};
 
 // ------------------------------------------------------- LDS row layout
 // Element m of row r lives at  r*RowStride<M> + m + (m >> 4).  The (m >> 4)
@@ -171,13 +175,18 @@ struct Lds {
 // ------------------------------------------------ radix plan for size M
 constexpr int log2c(long n) { return n <= 1 ? 0 : 1 + log2c(n / 2); }
 // radix of pass p for M = 2^m: as many radix-16 passes as possible, the
-// remainder (2, 4 or 8) first.
-template <int M>
+// remainder first: 4 or 8 as a pass of its own, a remainder of 2 merged
+// with one radix-16 pass into radix 32 (M = 32, 512, 8192 save a pass)
+// unless MERGE is false (a radix-32 butterfly holds 64 more registers).
+template <int M, bool MERGE = true>
 struct Plan {
   static constexpr int kLog = log2c(M);
   static constexpr int kRem = kLog % 4;
-  static constexpr int kPasses = kLog / 4 + (kRem ? 1 : 0);
-  static constexpr int radix(int p) { return (kRem && p == 0) ? (1 << kRem) : 16; }
+  static constexpr bool kMerge = MERGE && kRem == 1 && kLog > 1;
+  static constexpr int kPasses = kLog / 4 + (kRem && !kMerge ? 1 : 0);
+  static constexpr int radix(int p) {
+    return p == 0 ? (kMerge ? 32 : (kRem ? (1 << kRem) : 16)) : 16;
+  }
   static constexpr int ns(int p) {   // product of the radices before pass p
     int n = 1;
     for (int q = 0; q < p; ++q) n *= radix(q);
@@ -253,22 +262,23 @@ __device__ __forceinline__ void stockham_pass(float2* const (&buf)[NCH],
   __syncthreads();
 }
 
-template <int M, int ROWS, int DIR, int NT, int NCH, int P>
+template <int M, int ROWS, int DIR, int NT, int NCH, int P, bool MERGE>
 __device__ __forceinline__ void lds_fft_passes(float2* const (&buf)[NCH],
                                                const float2* __restrict__ tw) {
-  if constexpr (P < Plan<M>::kPasses) {
-    stockham_pass<M, ROWS, Plan<M>::radix(P), Plan<M>::ns(P), DIR, NT, NCH>(buf, tw);
-    lds_fft_passes<M, ROWS, DIR, NT, NCH, P + 1>(buf, tw);
+  using PL = Plan<M, MERGE>;
+  if constexpr (P < PL::kPasses) {
+    stockham_pass<M, ROWS, PL::radix(P), PL::ns(P), DIR, NT, NCH>(buf, tw);
+    lds_fft_passes<M, ROWS, DIR, NT, NCH, P + 1, MERGE>(buf, tw);
   }
 }
 
 // Batched FFT of ROWS rows of length M held in LDS (layout Lds<M>), on NCH
 // buffers.  Must be called by all NT threads; starts and ends with a barrier.
-template <int M, int ROWS, int DIR, int NT, int NCH>
+template <int M, int ROWS, int DIR, int NT, int NCH, bool MERGE = true>
 __device__ __forceinline__ void lds_fft(float2* const (&buf)[NCH],
                                         const float2* __restrict__ tw) {
   __syncthreads();
-  lds_fft_passes<M, ROWS, DIR, NT, NCH, 0>(buf, tw);
+  lds_fft_passes<M, ROWS, DIR, NT, NCH, 0, MERGE>(buf, tw);
 }
 
 // ---------------------------------------------------------- exact phases
