@@ -175,16 +175,19 @@ struct Lds {
 // ------------------------------------------------ radix plan for size M
 constexpr int log2c(long n) { return n <= 1 ? 0 : 1 + log2c(n / 2); }
 // radix of pass p for M = 2^m: as many radix-16 passes as possible, the
-// remainder first: 4 or 8 as a pass of its own, a remainder of 2 merged
-// with one radix-16 pass into radix 32 (M = 32, 512, 8192 save a pass)
-// unless MERGE is false (a radix-32 butterfly holds 64 more registers).
+// remainder first: 8 as a pass of its own; a remainder of 2 merged with one
+// radix-16 pass into radix 32 (M = 32, 512, 8192 save a pass), a remainder
+// of 4 with two radix-16 passes into 32 x 32 (M = 1024 saves a pass), unless
+// MERGE is false (a radix-32 butterfly holds 64 more registers).
 template <int M, bool MERGE = true>
 struct Plan {
   static constexpr int kLog = log2c(M);
   static constexpr int kRem = kLog % 4;
-  static constexpr bool kMerge = MERGE && kRem == 1 && kLog > 1;
-  static constexpr int kPasses = kLog / 4 + (kRem && !kMerge ? 1 : 0);
+  static constexpr bool kMerge = MERGE && kRem == 1 && kLog > 1;      // 32, 16, ...
+  static constexpr bool kMerge2 = MERGE && kRem == 2 && kLog >= 10;   // 32, 32, 16, ...
+  static constexpr int kPasses = kLog / 4 + (kRem && !kMerge && !kMerge2 ? 1 : 0);
   static constexpr int radix(int p) {
+    if (kMerge2) return p < 2 ? 32 : 16;
     return p == 0 ? (kMerge ? 32 : (kRem ? (1 << kRem) : 16)) : 16;
   }
   static constexpr int ns(int p) {   // product of the radices before pass p
