@@ -1,6 +1,6 @@
 // Device-side FFT building blocks for gfx950 (CDNA4, wave64).
 //
-// Everything here is compile-time sized: register DFTs of radix 2..16 with
+// Everything here is compile-time sized: register DFTs of radix 2..32 with
 // constant-folded twiddles, and a workgroup-wide Stockham autosort FFT over
 // rows held in LDS.  Rows are stored with a padded layout so that the strided
 // accesses of the early passes and of the transposed read-out do not pile
