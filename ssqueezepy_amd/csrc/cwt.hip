@@ -256,17 +256,20 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
   const float invL = 1.0f / (float)L;   // a power of two: scaling is exact
   const Sig sg = sig_at(a, b);
 
-  // band samples into row 0 at their cyclic positions (the table is zero
-  // past the Nyquist bin)
   constexpr int IM = (M + kNT - 1) / kNT;
-  {
-    float2 v[IM];
+  constexpr int IR = (ROWS * M + kNT - 1) / kNT;
+  if constexpr (ROWS == 1) {
+    // one phase per workgroup: band samples times their phase ramp
+    // e(k a0 / L) (k = ka + m, reduced mod L, a divisor of 2^32, so 32-bit
+    // wrap-around is exact) straight to their cyclic positions
+    float2 v[IM], ph[IM];
     float pv[IM];
 #pragma unroll
     for (int it = 0; it < IM; ++it) {
       const int m = threadIdx.x + it * kNT;
       v[it] = xh[min(ka + m, half)];
       pv[it] = ps[min(m, M - 1)];
+      ph[it] = phase_tab((uint32_t)(ka + m) * (uint32_t)a0, a.ptab, L, a.pS);
     }
 #pragma unroll
     for (int it = 0; it < IM; ++it) {
@@ -275,17 +278,37 @@ __device__ __forceinline__ void band_task(const CwtArgs& a, const int4 t, const 
         float2 X, Xd;
         band_value(a, v[it], pv[it] * invL, ka + m, X, Xd);
         const int j = (ka + m) & (M - 1);
-        sW[Lds<M>::idx(0, j)] = X;
-        sD[Lds<M>::idx(0, j)] = Xd;
+        sW[Lds<M>::idx(0, j)] = cmul(X, ph[it]);
+        sD[Lds<M>::idx(0, j)] = cmul(Xd, ph[it]);
       }
     }
-  }
-  __syncthreads();
-  // phase ramps e(k (a0 + r) / L) of position j (bin k = ka + ((j - ka) mod M)):
-  // rows r >= 1 first, then row 0 in place.  k (a0 + r) is reduced mod L, a
-  // divisor of 2^32, so 32-bit wrap-around is exact.
-  constexpr int IR = (ROWS * M + kNT - 1) / kNT;
-  {
+  } else {
+    // band samples into row 0 at their cyclic positions (the table is zero
+    // past the Nyquist bin)
+    {
+      float2 v[IM];
+      float pv[IM];
+#pragma unroll
+      for (int it = 0; it < IM; ++it) {
+        const int m = threadIdx.x + it * kNT;
+        v[it] = xh[min(ka + m, half)];
+        pv[it] = ps[min(m, M - 1)];
+      }
+#pragma unroll
+      for (int it = 0; it < IM; ++it) {
+        const int m = threadIdx.x + it * kNT;
+        if (m < M) {
+          float2 X, Xd;
+          band_value(a, v[it], pv[it] * invL, ka + m, X, Xd);
+          const int j = (ka + m) & (M - 1);
+          sW[Lds<M>::idx(0, j)] = X;
+          sD[Lds<M>::idx(0, j)] = Xd;
+        }
+      }
+    }
+    __syncthreads();
+    // phase ramps e(k (a0 + r) / L) of position j (bin k = ka + ((j - ka) mod
+    // M)): rows r >= 1 first, then row 0 in place
     float2 ph[IR];
 #pragma unroll
     for (int it = 0; it < IR; ++it) {
