@@ -202,6 +202,8 @@ def main():
                     help='also run the plan autotuner and time the tuned plan')
     ap.add_argument('--no-batched', action='store_true',
                     help='skip the uneven-length batch comparison')
+    ap.add_argument('--no-stft', action='store_true',
+                    help='skip the ssq_stft line')
     ap.add_argument('--no-backward', action='store_true',
                     help='skip the forward+backward timing')
     ap.add_argument('--copy-calibration', action='store_true',
@@ -360,7 +362,8 @@ def main():
     if fb is not None:
         res["fwd_bwd"] = fb
         res["fwd_bwd_batch"] = fwd_bwd_batch(torch, sq)
-    res["ssq_stft"] = ssq_stft_line(torch, sq)
+    if not args.no_stft:
+        res["ssq_stft"] = ssq_stft_line(torch, sq)
     if not args.no_batched:
         res["batched_uneven"] = batched_uneven(torch, sq)
     if args.autotune:
