@@ -706,10 +706,21 @@ __global__ __launch_bounds__(kNT) void k_fwd_small(FwdArgs f, const float2* __re
   for (int k = threadIdx.x; k <= L / 2; k += kNT) xh[k] = s[Lds<L>::idx(0, k)];
 }
 
+// The forward four-step runs before the band classes can start, so its
+// workgroups are small (rows of at least kPX points each, 16 points per
+// thread): enough of them to spread over the CUs.
+constexpr int kPX = 1024;
+__host__ __device__ constexpr int fx_rows(int M) { return kPX / M > 1 ? kPX / M : 1; }
+__host__ __device__ constexpr int fx_threads(int M) {
+  return fx_rows(M) * M / 16 > 64 ? fx_rows(M) * M / 16 : 64;
+}
+
 template <int L2>
-__global__ __launch_bounds__(kNT) void k_fwd_A(FwdArgs f, const float2* __restrict__ tw, int L1,
-                                               float2* __restrict__ scr) {
-  constexpr int G = kP / L2;
+__global__ __launch_bounds__(fx_threads(L2)) void k_fwd_A(FwdArgs f,
+                                                          const float2* __restrict__ tw,
+                                                          int L1, float2* __restrict__ scr) {
+  constexpr int G = fx_rows(L2);
+  constexpr int kNT = fx_threads(L2);
   __shared__ float2 s[Lds<L2>::elems(G)];
   const int b = blockIdx.y;
   const int k1_0 = blockIdx.x * G;
@@ -730,9 +741,12 @@ __global__ __launch_bounds__(kNT) void k_fwd_A(FwdArgs f, const float2* __restri
 }
 
 template <int L1>
-__global__ __launch_bounds__(kNT) void k_fwd_B(FwdArgs f, const float2* __restrict__ tw, int L2,
-                                               const float2* __restrict__ scr) {
-  constexpr int G = kP / L1;
+__global__ __launch_bounds__(fx_threads(L1)) void k_fwd_B(FwdArgs f,
+                                                          const float2* __restrict__ tw,
+                                                          int L2,
+                                                          const float2* __restrict__ scr) {
+  constexpr int G = fx_rows(L1);
+  constexpr int kNT = fx_threads(L1);
   __shared__ float2 s[Lds<L1>::elems(G)];
   const int b = blockIdx.y;
   const int n2_0 = blockIdx.x * G;
@@ -874,7 +888,7 @@ int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, floa
   switch (L2) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_fwd_A<m>, dim3(L1 / (kP / m), batch), dim3(kNT), 0, st, f,  \
+    hipLaunchKernelGGL(k_fwd_A<m>, dim3(L1 / fx_rows(m), batch), dim3(fx_threads(m)), 0, st, f, \
                        twA, L1, scr);                           \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
@@ -886,7 +900,7 @@ int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, floa
   switch (L1) {
 #define C(m)                                                                       \
   case m:                                                                          \
-    hipLaunchKernelGGL(k_fwd_B<m>, dim3(L2 / (kP / m), batch), dim3(kNT), 0, st, f,  \
+    hipLaunchKernelGGL(k_fwd_B<m>, dim3(L2 / fx_rows(m), batch), dim3(fx_threads(m)), 0, st, f, \
                        twB, L2, scr);                           \
     break;
     C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
@@ -2371,11 +2385,14 @@ __global__ __launch_bounds__(kNT) void k_adj_ifft_small(const float2* __restrict
 }
 
 template <int L2>
-__global__ __launch_bounds__(kNT) void k_adj_ifftA(const float2* __restrict__ S,
-                                                   const float2* __restrict__ tw, int L1,
-                                                   const float2* __restrict__ ptab, int pS,
-                                                   float2* __restrict__ scr) {
-  constexpr int G = kP / L2;
+__global__ __launch_bounds__(fx_threads(L2)) void k_adj_ifftA(const float2* __restrict__ S,
+                                                              const float2* __restrict__ tw,
+                                                              int L1,
+                                                              const float2* __restrict__ ptab,
+                                                              int pS,
+                                                              float2* __restrict__ scr) {
+  constexpr int G = fx_rows(L2);
+  constexpr int kNT = fx_threads(L2);
   __shared__ float2 s[Lds<L2>::elems(G)];
   const int b = blockIdx.y;
   const int k1_0 = blockIdx.x * G;
@@ -2395,10 +2412,11 @@ __global__ __launch_bounds__(kNT) void k_adj_ifftA(const float2* __restrict__ S,
 }
 
 template <int L1>
-__global__ __launch_bounds__(kNT) void k_adj_ifftB(const float2* __restrict__ scr,
-                                                   const float2* __restrict__ tw, int L2,
-                                                   float* __restrict__ xp) {
-  constexpr int G = kP / L1;
+__global__ __launch_bounds__(fx_threads(L1)) void k_adj_ifftB(const float2* __restrict__ scr,
+                                                              const float2* __restrict__ tw,
+                                                              int L2, float* __restrict__ xp) {
+  constexpr int G = fx_rows(L1);
+  constexpr int kNT = fx_threads(L1);
   __shared__ float2 s[Lds<L1>::elems(G)];
   const int b = blockIdx.y;
   const int n2_0 = blockIdx.x * G;
@@ -2696,12 +2714,12 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
     float2* scr = S + (size_t)batch * L;
     {
       SSQK_GET_TW(tw, L2);
-      const dim3 g(L1 / (kP / L2), (unsigned)batch);
+      const dim3 g(L1 / fx_rows(L2), (unsigned)batch);
       switch (L2) {
 #define C(m)                                                                          \
   case m:                                                                             \
-    hipLaunchKernelGGL(k_adj_ifftA<m>, g, dim3(kNT), 0, st, S, tw, L1, a.ptab, a.pS,   \
-                       scr);                                                          \
+    hipLaunchKernelGGL(k_adj_ifftA<m>, g, dim3(fx_threads(m)), 0, st, S, tw, L1, a.ptab, \
+                       a.pS, scr);                                                    \
     break;
         C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096) C(8192)
 #undef C
@@ -2713,11 +2731,11 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
     }
     {
       SSQK_GET_TW(tw, L1);
-      const dim3 g(L2 / (kP / L1), (unsigned)batch);
+      const dim3 g(L2 / fx_rows(L1), (unsigned)batch);
       switch (L1) {
 #define C(m)                                                                          \
   case m:                                                                             \
-    hipLaunchKernelGGL(k_adj_ifftB<m>, g, dim3(kNT), 0, st, scr, tw, L2, xp);          \
+    hipLaunchKernelGGL(k_adj_ifftB<m>, g, dim3(fx_threads(m)), 0, st, scr, tw, L2, xp); \
     break;
         C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096) C(8192)
 #undef C
