@@ -59,6 +59,22 @@ const float2* twiddles(int M) {
   return d;
 }
 
+int allow_dynamic_lds(const void* fn, int bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int> done;
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(dev, fn);
+  auto it = done.find(key);
+  if (it != done.end() && it->second >= bytes) return SSQK_OK;
+  const hipError_t e =
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return check_hip(e, "hipFuncSetAttribute");
+  done[key] = bytes;
+  return SSQK_OK;
+}
+
 const float2* phase_table(int L) {
   static std::mutex mu;
   static std::map<std::pair<int, int>, float2*> cache;
@@ -410,13 +426,9 @@ int squeeze_impl(const void* Wx, const uint16_t* bins, const float* cst, int64_t
   int W = 64;
   while (W > 8 && !fits(nf, W)) W /= 2;
   const int64_t R = fits(nf, W) ? nf : (int64_t)(kLds / ((W + 1) * sizeof(float2))) - 4;
-  static bool attr_set[4] = {};
-  auto launch = [&](auto kern, int wi) -> int {
-    if (!attr_set[wi]) {
-      SSQK_CHECK(hipFuncSetAttribute((const void*)kern,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds));
-      attr_set[wi] = true;
-    }
+  auto launch = [&](auto kern) -> int {
+    const int rc = ssqk_host::allow_dynamic_lds((const void*)kern, (int)kLds);
+    if (rc) return rc;
     for (int64_t r0 = 0; r0 < nf; r0 += R) {
       const int64_t rows = std::min<int64_t>(R, nf - r0);
       const size_t lds = (size_t)(rows + 4) * (W + 1) * sizeof(float2);
@@ -429,10 +441,10 @@ int squeeze_impl(const void* Wx, const uint16_t* bins, const float* cst, int64_t
     }
     return SSQK_OK;
   };
-  if (W == 64) return launch(k_squeeze<64>, 0);
-  if (W == 32) return launch(k_squeeze<32>, 1);
-  if (W == 16) return launch(k_squeeze<16>, 2);
-  return launch(k_squeeze<8>, 3);
+  if (W == 64) return launch(k_squeeze<64>);
+  if (W == 32) return launch(k_squeeze<32>);
+  if (W == 16) return launch(k_squeeze<16>);
+  return launch(k_squeeze<8>);
 }
 }  // namespace
 

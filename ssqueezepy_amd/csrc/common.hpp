@@ -157,6 +157,9 @@ int check_hip(hipError_t e, const char* what);
 const float2* twiddles(int M);
 // Two-level phase table of ssqk::phase_tab for length L (power of two).
 const float2* phase_table(int L);
+// Lets kernel `fn` use up to `bytes` of dynamic LDS on the current device
+// (set once per device and kernel; thread-safe).  SSQK_OK or an error.
+int allow_dynamic_lds(const void* fn, int bytes);
 }  // namespace ssqk_host
 
 // twiddle table of size M into `var`, failing the entry point if it could

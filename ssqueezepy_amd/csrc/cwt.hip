@@ -2556,20 +2556,16 @@ int adjoint_impl(ssqk_cwt_plan_s* p, const float2* gTx, const float2* gWx,
     const size_t lds = (size_t)nf * W * sizeof(float2);
     dim3 grid(xcd_grid((unsigned)((N + W - 1) / W)), (unsigned)batch);
     float2* yb = (float2*)(ws + off[0]);
-    static bool attr[3] = {false, false, false};
-    auto go = [&](auto kern, int wi) -> int {
-      if (!attr[wi]) {
-        SSQK_CHECK(hipFuncSetAttribute((const void*)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds));
-        attr[wi] = true;
-      }
+    auto go = [&](auto kern) -> int {
+      const int rc = ssqk_host::allow_dynamic_lds((const void*)kern, (int)kLds);
+      if (rc) return rc;
       hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, gTx, gWx, bins, cst, (int)p->na,
                          (int)nf, N, yb);
       SSQK_CHECK(hipGetLastError());
       return SSQK_OK;
     };
-    const int rc = W == 32 ? go(k_adj_ybar<32>, 0) : W == 16 ? go(k_adj_ybar<16>, 1)
-                                                             : go(k_adj_ybar<8>, 2);
+    const int rc = W == 32 ? go(k_adj_ybar<32>) : W == 16 ? go(k_adj_ybar<16>)
+                                                          : go(k_adj_ybar<8>);
     if (rc) return rc;
   }
   // band classes, the four-step chunks and the overlap-save classes are
