@@ -38,7 +38,7 @@ const float2* twiddles(int M) {
   static std::mutex mu;
   static std::map<std::pair<int, int>, float2*> cache;
   int dev = 0;
-  hipGetDevice(&dev);
+  (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lock(mu);
   auto key = std::make_pair(dev, M);
   auto it = cache.find(key);
@@ -63,7 +63,7 @@ int allow_dynamic_lds(const void* fn, int bytes) {
   static std::mutex mu;
   static std::map<std::pair<int, const void*>, int> done;
   int dev = 0;
-  hipGetDevice(&dev);
+  (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lock(mu);
   const auto key = std::make_pair(dev, fn);
   auto it = done.find(key);
@@ -79,7 +79,7 @@ const float2* phase_table(int L) {
   static std::mutex mu;
   static std::map<std::pair<int, int>, float2*> cache;
   int dev = 0;
-  hipGetDevice(&dev);
+  (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lock(mu);
   auto key = std::make_pair(dev, L);
   auto it = cache.find(key);

@@ -1488,7 +1488,6 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       std::vector<int4> ag;
       std::vector<int64_t> ab;
       int64_t rows = 0;
-      size_t gfirst = 0;
       for (size_t c = 0; c < p->os.size(); ++c) {
         p->adj_os_cls_off.push_back(rows);
         const int F = p->os[c].F;
@@ -1502,7 +1501,6 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
           task0 += nblk;
         }
         rows += task0 * F;
-        gfirst += ng;
       }
       p->adj_os_ngroups = (int)ag.size();
       p->adj_os_rows = rows;
@@ -2160,7 +2158,7 @@ template <int F>
 __global__ __launch_bounds__(AdjOsShape<F>::NT) void k_adj_os(AdjArgs a,
                                                               const OsGroup* __restrict__ groups,
                                                               const int2* __restrict__ tasks,
-                                                              int ntasks,
+                                                              int /*ntasks*/,
                                                               const float2* __restrict__ coef,
                                                               int coef_stride,
                                                               const int2* __restrict__ coef_band,
