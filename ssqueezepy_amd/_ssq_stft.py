@@ -54,7 +54,9 @@ def ssq_stft(x, window=None, n_fft=None, win_len=None, hop_len=1, fs=None, t=Non
     const = float(ssq_freqs[1] - ssq_freqs[0])
 
     Sx = torch.empty(shape, dtype=torch.complex64, device=dev)
-    dSx = torch.empty_like(Sx)
+    # dSx is stored only when returned or needed by the get_w route; the
+    # fused bins use it in registers
+    dSx = torch.empty_like(Sx) if (get_dWx or get_w) else None
     bins = torch.empty(shape, dtype=torch.int16, device=dev)
     win = device_constant(window, torch.float32, dev)
     dwin = device_constant(diff_window, torch.float32, dev)
@@ -71,7 +73,7 @@ def ssq_stft(x, window=None, n_fft=None, win_len=None, hop_len=1, fs=None, t=Non
                        Sx, dSx, Sfs=Sfs_t, bins=bins, bp=bp, gamma=gamma)
     Ssq = _apply_squeezing(Sx, squeezing).contiguous()
     Tx = torch.empty_like(Sx)
-    cst = torch.full((rows,), const, dtype=torch.float32, device=dev)
+    cst = device_constant(np.full(rows, const, dtype=np.float32), torch.float32, dev)
     _lib.squeeze(Ssq, bins, cst, rows, Tx)
     if flipud:
         ssq_freqs = ssq_freqs[::-1]

@@ -78,6 +78,27 @@ def _check_NOLA(window, hop_len, dtype=None):
              "Lower `hop_len`, choose wider `window`, or use `dtype='float64'`.")
 
 
+_WINDOWS = {}
+
+
+def _windows(window, win_len, n_fft, hop_len, dtype):
+    """get_window + the NOLA checks, memoized for windows given by name (or
+    None): they depend only on these arguments."""
+    key = None
+    if window is None or isinstance(window, (str, tuple)):
+        key = (window, win_len, n_fft, hop_len, dtype)
+        hit = _WINDOWS.get(key)
+        if hit is not None:
+            return hit
+    w, dw = get_window(window, win_len, n_fft, derivative=True, dtype=dtype)
+    _check_NOLA(w, hop_len, dtype)
+    if key is not None:
+        if len(_WINDOWS) > 64:
+            _WINDOWS.clear()
+        _WINDOWS[key] = (w, dw)
+    return w, dw
+
+
 def stft_setup(x, window, n_fft, win_len, hop_len, fs, t, padtype, dtype):
     N = x.shape[-1]
     _, fs, _ = _process_fs_and_t(fs, t, N)
@@ -85,9 +106,7 @@ def stft_setup(x, window, n_fft, win_len, hop_len, fs, t, padtype, dtype):
     if win_len is None:
         win_len = len(window) if isinstance(window, np.ndarray) else n_fft
     dtype = gdefaults('_stft.stft', dtype=dtype)['dtype']
-    window, diff_window = get_window(window, win_len, n_fft, derivative=True,
-                                     dtype=dtype)
-    _check_NOLA(window, hop_len, dtype)
+    window, diff_window = _windows(window, win_len, n_fft, hop_len, dtype)
     if padtype not in ('reflect', 'zero'):
         raise NotImplementedError("stft padtype must be 'reflect' or 'zero' on "
                                   "the GPU path (got %s)" % padtype)
