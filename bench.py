@@ -159,6 +159,58 @@ def ssq_stft_line(torch, sq, N=160000, n_fft=512, steps=20):
             "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def hyperbolic_tones_noise(N, seed):
+    """Hyperbolic chirp + two tones (0.17, 0.31 cycles/sample) + noise
+    sigma 0.2 (float32)."""
+    n = np.arange(N, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    t = n / N
+    x = (np.cos(2 * np.pi * 200 * np.log1p(9 * t) / np.log(10))
+         + 0.6 * np.cos(2 * np.pi * 0.17 * n) + 0.4 * np.cos(2 * np.pi * 0.31 * n)
+         + 0.2 * rng.standard_normal(N))
+    return x.astype(np.float32)
+
+
+def large_n(torch, sq, B=8, N=1 << 20, steps=3):
+    """8 signals N = 2^20, GMW(3,60), nv=32: forward, and forward + backward
+    of loss mean(||Tx| - |Tx_target||^2) (target: seeds 100..107); the
+    backward keeps only the forward's uint16 squeeze rows, no Wx."""
+    dev = torch.device('cuda', 0)
+    x = torch.as_tensor(np.stack([hyperbolic_tones_noise(N, s) for s in range(B)]), device=dev)
+    with torch.no_grad():
+        tgt = sq.ssq_cwt(torch.as_tensor(np.stack([hyperbolic_tones_noise(N, 100 + s)
+                                                   for s in range(B)]), device=dev))[0].abs()
+
+    def fwd():
+        return sq.ssq_cwt(x)[0]
+
+    def fwd_bwd():
+        xg = x.detach().requires_grad_(True)
+        T = sq.ssq_cwt(xg)[0]
+        ((T.abs() - tgt) ** 2).mean().backward()
+        return xg.grad
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    ms_f = timed(fwd)
+    ms_fb = timed(fwd_bwd)
+    na = int(tgt.shape[1])
+    del tgt
+    return {"workload": "%d signals N=%d (hyperbolic chirp + 2 tones + noise), GMW(3,60), "
+                        "nv=32 (na=%d)" % (B, N, na),
+            "ms_forward": round(ms_f, 3), "ms_fwd_bwd": round(ms_fb, 3),
+            "ratio_to_forward": round(ms_fb / ms_f, 3),
+            "value_forward": round(B * N * na / (ms_f * 1e-3) / 1e9, 3),
+            "note": "ms_fwd_bwd includes the loss and its torch backward"}
+
+
 def _traffic():
     """HBM bytes per k_squeeze launch measured with rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy), kept in
@@ -202,6 +254,8 @@ def main():
                     help='also run the plan autotuner and time the tuned plan')
     ap.add_argument('--no-batched', action='store_true',
                     help='skip the uneven-length batch comparison')
+    ap.add_argument('--large', action='store_true',
+                    help='add the 8 x 2^20 forward and forward+backward line')
     ap.add_argument('--no-stft', action='store_true',
                     help='skip the ssq_stft line')
     ap.add_argument('--no-backward', action='store_true',
@@ -364,6 +418,8 @@ def main():
         res["fwd_bwd_batch"] = fwd_bwd_batch(torch, sq)
     if not args.no_stft:
         res["ssq_stft"] = ssq_stft_line(torch, sq)
+    if args.large:
+        res["large_n"] = large_n(torch, sq)
     if not args.no_batched:
         res["batched_uneven"] = batched_uneven(torch, sq)
     if args.autotune:
