@@ -178,14 +178,17 @@ __global__ void k_bins_w(const float* __restrict__ w, int64_t total, BinDev bp,
 // -------------------------------------------------------------- squeeze
 // One workgroup owns W consecutive columns of one signal and a window of
 // output rows [r0, r0 + R) for every scale; the accumulators are
-// interleaved complex [R + 4][W + 1] in LDS (odd row stride; 4 dummy rows).
+// interleaved complex in LDS, in one of two layouts: PAD, [R + 4][W + 1]
+// (odd row stride, 4 dummy rows), or compact, [R][W] with column c of row r
+// at c ^ (r & 31), which holds 64 columns of up to 320 rows.
 // Eight lanes serve one column: two quads, each lane loading its own scale,
 // so one load instruction covers 8 columns x 8 consecutive scales.  Scales
 // are taken in groups of 8: quad p holds scales 4p..4p+3 of the group.
 // Inside a quad the four (bin, value) pairs are exchanged by DPP
 // broadcasts; contributions that land in the same row are summed in
 // ascending scale order by the lowest such lane, the others target their
-// dummy row.  Quad 0 then quad 1 update LDS, so every accumulator receives
+// dummy row (PAD) or skip the update.  Quad 0 then quad 1 update LDS, so
+// every accumulator receives
 // its column's scales in ascending order: deterministic, no atomics, Tx
 // written once.  Two blocks of kSqUL groups are in flight alternately.
 constexpr int kSqQPC = 2;                    // quads per column
@@ -203,15 +206,17 @@ __device__ __forceinline__ float quad_bcast(float v) {
   return __int_as_float(quad_bcast<S>(__float_as_int(v)));
 }
 
-template <int W>
+template <int W, bool PAD>
 __global__ __launch_bounds__(W * 4 * kSqQPC)
 void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
                const float* __restrict__ cst, int na, int r0, int R, int64_t Ngrid,
                int64_t in_stride, int64_t out_stride, float2* __restrict__ Tx,
                const SigDev* __restrict__ sig) {
   constexpr int NT = W * 4 * kSqQPC;
-  constexpr int RS = W + 1;
-  extern __shared__ float2 s_acc[];          // [R + 4][RS]
+  constexpr int RS = PAD ? W + 1 : W;        // row stride
+  constexpr int SW = (W < 32 ? W : 32) - 1;  // column swizzle mask (compact)
+  extern __shared__ float2 s_acc[];          // [R + 4][RS] or [R][RS]
+  auto at = [](int r, int cc) { return PAD ? r * RS + cc : r * RS + (cc ^ (r & SW)); };
   const int b = blockIdx.y;
   // XCD-aware tile order: block x goes to XCD x % 8, which takes a
   // contiguous share of the column tiles, so the bins line two neighbouring
@@ -233,7 +238,7 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
   const int64_t j0 = (int64_t)tile * W;
   if (j0 >= N) return;                       // (shorter signal of a ragged batch)
   const int tid = threadIdx.x;
-  for (int e = tid; e < (R + 4) * RS; e += NT) s_acc[e] = make_float2(0.f, 0.f);
+  for (int e = tid; e < (R + (PAD ? 4 : 0)) * RS; e += NT) s_acc[e] = make_float2(0.f, 0.f);
   __syncthreads();
 
   const int lane = tid & 63, wave = tid >> 6;
@@ -246,7 +251,6 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
   const uint32_t offL = (uint32_t)sl * (uint32_t)N + cl;
   const float2* Wb = Wx + ioff + j0;
   const uint16_t* Bb = bins + ioff + j0;
-  const int dummy = R + q;
 
   // rows past na re-read row ib (in bounds) and are masked below
   auto load = [&](int i0, float2* w, int* k, float* cc) {
@@ -280,14 +284,23 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
       if (q <= 1 && k1 == k) { sx += x1; sy += y1; }
       if (q <= 2 && k2 == k) { sx += x2; sy += y2; }
       if (k3 == k) { sx += x3; sy += y3; }
-      const int row = (dup || k < 0) ? dummy : k;
+      const bool mine = !dup && k >= 0;
 #pragma unroll
       for (int pp = 0; pp < kSqQPC; ++pp) {
-        float2* acc_p = s_acc + (p == pp ? row : dummy) * RS + c;
-        float2 acc = *acc_p;
-        acc.x += sx;
-        acc.y += sy;
-        *acc_p = acc;
+        if constexpr (PAD) {
+          const int dummy = R + q;
+          float2* acc_p = s_acc + (p == pp && mine ? k : dummy) * RS + c;
+          float2 acc = *acc_p;
+          acc.x += sx;
+          acc.y += sy;
+          *acc_p = acc;
+        } else if (p == pp && mine) {
+          float2* acc_p = s_acc + at(k, c);
+          float2 acc = *acc_p;
+          acc.x += sx;
+          acc.y += sy;
+          *acc_p = acc;
+        }
       }
     }
   };
@@ -319,7 +332,7 @@ void k_squeeze(const float2* __restrict__ Wx, const uint16_t* __restrict__ bins,
     const int64_t jj = j0 + cc;
     // streaming stores: Tx bypasses the caches, which keep Wx and bins
     if (jj < N) {
-      const float2 v = s_acc[k * RS + cc];
+      const float2 v = s_acc[at(k, cc)];
       const unsigned long long bits = (unsigned long long)__float_as_uint(v.x) |
                                       ((unsigned long long)__float_as_uint(v.y) << 32);
       __builtin_nontemporal_store(bits,
@@ -416,22 +429,25 @@ namespace {
 int squeeze_impl(const void* Wx, const uint16_t* bins, const float* cst, int64_t batch,
                  int64_t na, int64_t nf, int64_t N, void* Tx, const SigDev* sig,
                  void* stream) {
-  // accumulators [R + 4][W + 1] complex in 160 KiB of LDS: 64 columns while
-  // all rows fit (nf <= 311), else 32, 16 or 8; taller outputs are done in
-  // row windows
+  // accumulators in 160 KiB of LDS: 64 columns in the padded layout while
+  // all rows fit (nf <= 311), 64 in the compact one up to nf = 320, else 32,
+  // 16 or 8 padded; taller outputs are done in row windows
   constexpr size_t kLds = 160 * 1024;
   auto fits = [&](int64_t rows, int w) {
     return (size_t)(rows + 4) * (w + 1) * sizeof(float2) <= kLds;
   };
+  const bool compact = !fits(nf, 64) && (size_t)nf * 64 * sizeof(float2) <= kLds;
   int W = 64;
-  while (W > 8 && !fits(nf, W)) W /= 2;
-  const int64_t R = fits(nf, W) ? nf : (int64_t)(kLds / ((W + 1) * sizeof(float2))) - 4;
+  while (!compact && W > 8 && !fits(nf, W)) W /= 2;
+  const int64_t R = compact || fits(nf, W)
+                        ? nf : (int64_t)(kLds / ((W + 1) * sizeof(float2))) - 4;
   auto launch = [&](auto kern) -> int {
     const int rc = ssqk_host::allow_dynamic_lds((const void*)kern, (int)kLds);
     if (rc) return rc;
     for (int64_t r0 = 0; r0 < nf; r0 += R) {
       const int64_t rows = std::min<int64_t>(R, nf - r0);
-      const size_t lds = (size_t)(rows + 4) * (W + 1) * sizeof(float2);
+      const size_t lds = compact ? (size_t)rows * W * sizeof(float2)
+                                 : (size_t)(rows + 4) * (W + 1) * sizeof(float2);
       const unsigned ntiles = (unsigned)((N + W - 1) / W);
       dim3 grid(kSqXcd ? (ntiles + 7) / 8 * 8 : ntiles, (unsigned)batch);
       hipLaunchKernelGGL(kern, grid, dim3(W * 4 * kSqQPC), lds, (hipStream_t)stream,
@@ -441,10 +457,11 @@ int squeeze_impl(const void* Wx, const uint16_t* bins, const float* cst, int64_t
     }
     return SSQK_OK;
   };
-  if (W == 64) return launch(k_squeeze<64>);
-  if (W == 32) return launch(k_squeeze<32>);
-  if (W == 16) return launch(k_squeeze<16>);
-  return launch(k_squeeze<8>);
+  if (compact) return launch(k_squeeze<64, false>);
+  if (W == 64) return launch(k_squeeze<64, true>);
+  if (W == 32) return launch(k_squeeze<32, true>);
+  if (W == 16) return launch(k_squeeze<16, true>);
+  return launch(k_squeeze<8, true>);
 }
 }  // namespace
 

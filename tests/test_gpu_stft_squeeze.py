@@ -114,14 +114,16 @@ def test_phase_entry_points_vs_reference(golden):
 @pytest.mark.parametrize('na,nf,N,batch', [(1, 1, 1, 1), (7, 5, 33, 2), (293, 293, 1000, 1),
                                            (300, 3000, 700, 1), (17, 2500, 50, 2),
                                            (9, 70, 4099, 3), (40, 400, 1029, 1),
-                                           (12, 1000, 70, 2)])
+                                           (12, 1000, 70, 2), (317, 317, 3000, 2),
+                                           (320, 320, 129, 1), (312, 312, 700, 1)])
 def test_squeeze_kernel_shapes(na, nf, N, batch):
     """ssqk_squeeze against the float64 brute-force scatter: partial column
     blocks, tile counts that are not a multiple of the 8 XCDs, scale counts
     that are not a multiple of the 8-scale groups, every tile width (64, 32,
-    16 and 8 columns by output height), empty bins, row windows (output rows
-    beyond one LDS tile), batches; bins clustered so many scales of a column
-    share a row."""
+    16 and 8 columns by output height) and both accumulator layouts (padded
+    up to 311 rows, compact for 312..320 rows at 64 columns), empty bins,
+    row windows (output rows beyond one LDS tile), batches; bins clustered
+    so many scales of a column share a row."""
     import torch
     from ssqueezepy_amd import _lib
     rng = np.random.default_rng(na * 7 + nf)
