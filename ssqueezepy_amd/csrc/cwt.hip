@@ -37,6 +37,7 @@ constexpr int kP = 8192;    // largest band transform of one workgroup
 constexpr int kNT = 512;    // threads of the single-workgroup forward FFT
 constexpr int kSmallFwd = 16384;  // largest L done by one workgroup
 constexpr int kPF = 4096;   // points per channel of a four-step workgroup
+constexpr int kOsMax = 8192;  // largest overlap-save block
 
 // Points per channel of a band-path workgroup: at least 2048 so that small
 // transforms batch many phases, at most M so that LDS stays <= 70 KB for
@@ -518,7 +519,7 @@ struct OsGroup {
 
 template <int F>
 struct OsShape {
-  static constexpr int SR = (4096 / F) > 8 ? 8 : (4096 / F);  // scales per batch
+  static constexpr int SR = F >= 4096 ? 1 : (4096 / F > 8 ? 8 : 4096 / F);  // scales per batch
   static constexpr int NT = (SR * F) / 8 < 64 ? 64 : (SR * F) / 8;
 };
 
@@ -533,8 +534,9 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
                                                  int pad) {
   constexpr int SR = OsShape<F>::SR;
   constexpr int NT = OsShape<F>::NT;
-  // two channels x SR rows in LDS (<= 70 KB); the segment spectrum stays in
-  // registers, in the layout of the multiply stage
+  // two channels x SR rows in LDS (<= 70 KB up to F = 4096, 139 KB at
+  // 8192); the segment spectrum stays in registers, in the layout of the
+  // multiply stage
   __shared__ float2 sA[Lds<F>::elems(SR)];
   __shared__ float2 sD[Lds<F>::elems(SR)];
   // (plain block order: measured faster here than the XCD-grouped order)
@@ -597,7 +599,9 @@ __global__ __launch_bounds__(OsShape<F>::NT) void k_cwt_os(CwtArgs a, const OsGr
       sD[Lds<F>::idx(r, k)] = ok ? cmul(Sreg[it], cD) : make_float2(0.f, 0.f);
     }
     float2* const bufs[2] = {sA, sD};
-    lds_fft<F, SR, +1, NT, 2>(bufs, tw);
+    // (at F = 8192, 1024 threads, a radix-32 pass would spill: the segment
+    // spectrum stays in registers across it)
+    lds_fft<F, SR, +1, NT, 2, (F < 8192)>(bufs, tw);
     // (SR * B <= SR * F = 8 NT, so a thread has at most 8 points)
     uint32_t edge = 0;
     auto emit_all = [&](auto plain) {
@@ -825,7 +829,8 @@ int launch_fullB(int L1, dim3 grid, hipStream_t st, const CwtArgs& a, const int4
   return SSQK_OK;
 }
 
-#define SSQK_OS_CASES(MACRO) MACRO(256) MACRO(512) MACRO(1024) MACRO(2048) MACRO(4096)
+#define SSQK_OS_CASES(MACRO) \
+  MACRO(256) MACRO(512) MACRO(1024) MACRO(2048) MACRO(4096) MACRO(8192)
 
 int launch_os(int F, dim3 grid, hipStream_t st, const CwtArgs& a, const OsGroup* groups,
               const int2* tasks, const float2* coef, int coef_stride, const int2* coef_band,
@@ -1229,8 +1234,16 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
     // kernel's time tail is long) and a block comfortably longer than 2H
     if (opt.allow_os && kb_f < (double)half - 1 && K > 1) {
       const int H = (int)std::ceil(T1 * s * 1.02) + 2;
-      for (int F = 256; F <= 4096; F *= 2) {
+      for (int F = 256; F <= kOsMax; F *= 2) {
         if (F < 2 * H + 64 || 2 * (int64_t)H + 1 > N) continue;
+        // the largest block (one 1024-thread workgroup per CU) runs slower
+        // than its estimate: against the band path, against smaller blocks
+        // (at L = 2^21, 169 us per scale for H = 313..2831 where 4096-point
+        // blocks took 117 us for H = 165..1480), and against the four-step
+        // path while that one's scratch stays in the caches (L = 2^18: 1.02
+        // ms against 0.93 per ssq_cwt call).  It only takes scales that no
+        // smaller block takes and the four-step path would, at L >= 2^20.
+        if (F == kOsMax && (bestF || M <= kP || L < ((int64_t)1 << 20))) continue;
         const double c = ((std::log2((double)F) + 3.0) * F / (double)(F - 2 * H) +
                           std::log2((double)F) / 8.0) * opt.os_bias;
         if (c < cost) { cost = c; bestF = F; bestH = H; }

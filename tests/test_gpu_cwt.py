@@ -149,6 +149,23 @@ def test_cwt_vs_oracle(N, kind):
     assert scales.dtype.itemsize == 4
 
 
+@pytest.mark.parametrize('padtype', ['zero', 'symmetric', 'replicate', 'wrap'])
+@pytest.mark.parametrize('N,kind', [(77, 'morlet'), (1000, 'gmw'), (6000, 'morlet')])
+def test_cwt_padtypes_vs_oracle(N, kind, padtype):
+    """Every pad mode of padsignal (utils/common.py:123-158) against the
+    oracle's np.pad, on the band, overlap-save and four-step paths (the pad
+    is applied where each kernel reads the signal)."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+    x = chirp_tone_noise(N, 5)
+    g = ssq_cwt_grid(N, _wav(kind))
+    print(f"cwt N={N} {kind} padtype={padtype}", flush=True)
+    Wx, scales, dWx = sq.cwt(x, _wav(kind), derivative=True, padtype=padtype)
+    Wx_o, dWx_o = O.cwt(x.astype(np.float64), _psih(kind), g['scales64'], padtype=padtype)
+    assert rel_l2(Wx.cpu().numpy(), Wx_o) < 1e-5
+    assert rel_l2(dWx.cpu().numpy(), dWx_o) < 1e-5
+
+
 @pytest.mark.parametrize('N,kind,get_w', [(300, 'morlet', True), (2048, 'gmw', False),
                                           (12000, 'gmw', True), (40000, 'gmw', False)])
 def test_ssq_cwt_vs_oracle(N, kind, get_w):
