@@ -96,7 +96,8 @@ def batched_uneven(torch, sq, n_signals=256, passes=3):
 
 def fwd_bwd_batch(torch, sq, B=32, N=16384, steps=10):
     """Training-style step on a batch of 32 signals N=16,384 (chirp + tone +
-    noise, seeds 0..31, GMW default): loss mean(||Tx| - |Tx_target||^2) with
+    noise, seeds 0..31, GMW default): loss mean(||Tx| - |Tx_target||^2)
+    (torch's fused mse_loss of |Tx|) with
     the target from seeds 100..131, gradient w.r.t. x through the adjoint.
     Forward alone, forward + backward, and their ratio."""
     dev = torch.device('cuda', 0)
@@ -111,7 +112,7 @@ def fwd_bwd_batch(torch, sq, B=32, N=16384, steps=10):
     def fwd_bwd():
         xg = x.detach().requires_grad_(True)
         T = sq.ssq_cwt(xg)[0]
-        ((T.abs() - tgt) ** 2).mean().backward()
+        torch.nn.functional.mse_loss(T.abs(), tgt).backward()
         return xg.grad
 
     def timed(fn):
@@ -128,7 +129,8 @@ def fwd_bwd_batch(torch, sq, B=32, N=16384, steps=10):
     ms_fb = timed(fwd_bwd)
     na = int(tgt.shape[1])
     return {"workload": "%d signals N=%d (chirp + tone + noise), GMW(3,60), nv=32 "
-                        "(na=%d); loss mean(||Tx| - |Tx_target||^2)" % (B, N, na),
+                        "(na=%d); loss mean(||Tx| - |Tx_target||^2) (torch mse_loss of |Tx|)"
+                        % (B, N, na),
             "ms_forward": round(ms_f, 4), "ms_fwd_bwd": round(ms_fb, 4),
             "ratio_to_forward": round(ms_fb / ms_f, 3),
             "value_forward": round(B * N * na / (ms_f * 1e-3) / 1e9, 3),
@@ -173,7 +175,8 @@ def hyperbolic_tones_noise(N, seed):
 
 def large_n(torch, sq, B=8, N=1 << 20, steps=3):
     """8 signals N = 2^20, GMW(3,60), nv=32: forward, and forward + backward
-    of loss mean(||Tx| - |Tx_target||^2) (target: seeds 100..107); the
+    of loss mean(||Tx| - |Tx_target||^2) (torch mse_loss of |Tx|; target:
+    seeds 100..107); the
     backward keeps only the forward's uint16 squeeze rows, no Wx."""
     dev = torch.device('cuda', 0)
     x = torch.as_tensor(np.stack([hyperbolic_tones_noise(N, s) for s in range(B)]), device=dev)
@@ -187,7 +190,7 @@ def large_n(torch, sq, B=8, N=1 << 20, steps=3):
     def fwd_bwd():
         xg = x.detach().requires_grad_(True)
         T = sq.ssq_cwt(xg)[0]
-        ((T.abs() - tgt) ** 2).mean().backward()
+        torch.nn.functional.mse_loss(T.abs(), tgt).backward()
         return xg.grad
 
     def timed(fn):
@@ -344,7 +347,7 @@ def main():
         def fwd_bwd():
             xg = x_dev.detach().requires_grad_(True)
             T = sq.ssq_cwt(xg, wav)[0]
-            ((T.abs() - tgt) ** 2).mean().backward()
+            torch.nn.functional.mse_loss(T.abs(), tgt).backward()
             return xg.grad
 
         for _ in range(2):
@@ -359,7 +362,7 @@ def main():
         # the adjoint alone (what the autograd backward of ssq_cwt runs),
         # on the gradient of the same loss w.r.t. Tx
         Tq = out[0].detach().requires_grad_(True)
-        ((Tq.abs() - tgt) ** 2).mean().backward()
+        torch.nn.functional.mse_loss(Tq.abs(), tgt).backward()
         gT = Tq.grad.contiguous()
         bins_a = bins
         gx = torch.empty_like(x_dev)
@@ -375,7 +378,8 @@ def main():
               "adjoint_ms": round(ms_adj, 4),
               "ratio_forward_plus_adjoint": round((ms_step + ms_adj) / ms_step, 3),
               "steps": fb_steps,
-              "loss": "mean(||Tx| - |Tx_target||^2), Tx_target from seed 1; gradient "
+              "loss": "mean(||Tx| - |Tx_target||^2) (torch mse_loss of |Tx|), Tx_target "
+                      "from seed 1; gradient "
                       "w.r.t. x by the adjoint (ssqk_cwt_adjoint); ms_per_step includes "
                       "the loss and its torch backward"}
     paths = dict(zip(('band', 'overlap_save', 'four_step'), plan.paths()))
