@@ -214,6 +214,29 @@ int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t n_fft,
                    const float* Sfs, uint16_t* bins, const ssqk_bin_params* bp,
                    double gamma, void* stream);
 
+/* ------------------------------------------------------------- inverses
+ * Column sums of the real part of A [batch, rows, N] complex64, rows
+ * r0 <= i < r1 in ascending order in float32 (NumPy's axis-0 order):
+ *   x[b][j] = f32(f32(scale * sum_i Re A[b][i][j] / div[i - r0]) + add)
+ * plus the previous x[b][j] (float32) when `accumulate`; div (float32
+ * [r1 - r0]) may be NULL for no division.  Replaces the full inversions
+ * `Tx.real.sum(axis=0) * 2 / Css` of issq_cwt (_ssq_cwt.py:364-378) and
+ * `* 2 / window[n//2]` of issq_stft (_ssq_stft.py:192-198), and the
+ * one-integral icwt `(Wx.real / norm(scales)).sum(axis=-2)` with its
+ * admissibility factor and x_mean (_cwt.py:437-482). */
+int ssqk_real_rowsum(const void* A, int64_t batch, int64_t rows, int64_t N,
+                     int64_t r0, int64_t r1, const float* div, double scale,
+                     double add, int accumulate, float* x, void* stream);
+/* Inversion over curve bands (_invert_components, _ssq_cwt.py:381-403):
+ * cc, cw int32 [N][K] (curve centre rows and half-widths per column;
+ * cc = -1: no curve there).  Component n of column j covers rows
+ * clip(cc - cw, 0, rows) .. clip(cc + cw, 0, rows); x float64 [K + 1][N]:
+ * x[n][j] = scale * (float64 sum of Re A over the band), x[K][j] = scale *
+ * (float32 sum over the rows outside every band), rows ascending. */
+int ssqk_real_bandsum(const void* A, int64_t rows, int64_t N, const int32_t* cc,
+                      const int32_t* cw, int64_t K, double scale, double* x,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

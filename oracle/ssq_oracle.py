@@ -22,6 +22,13 @@ ssqueezepy package root).  How each function is pinned:
                     (ssqueeze_fast / indexed_sum_onfly outputs)
   stft              direct per-frame DFT definition; golden e2e Sx/dSx
   ssq_cwt, ssq_stft golden e2e fixtures (Tx, Wx, w, dWx) and C1 samples
+  gmw_adm_ssq,
+  gmw_adm_cwt       closed forms (Gamma-function integrals of the GMW)
+  issq_full,
+  invert_components,
+  icwt_1int         the reference's reconstruction thresholds
+                    (tests/reconstruction_test.py:65-123) on the float64
+                    chain; components + remainder = full inversion
 """
 import numpy as np
 
@@ -253,3 +260,68 @@ def ssq_stft(x, window, diff_window, n_fft, gamma, hop=1, fs=1.,
     k = squeeze_bins(wf, params, rows - 1, flipud)
     Tx = ssqueeze(Sx, k, Sfs[1] - Sfs[0])
     return Tx, Sx, dSx, w, k, Sfs
+
+
+# --------------------------------------------------------------- inverses
+def gmw_adm_ssq(gamma=3., beta=60.):
+    """integral(psih(w) / w, 0..inf) of the L1-normed GMW (utils/cwt_utils.py:
+    28-47 integrates it numerically): 2 e^(wc^g) wc^(-b) Gamma(b/g) / g."""
+    from scipy.special import gammaln
+    wc = (beta / gamma)**(1 / gamma)
+    return 2 * np.exp(wc**gamma - beta * np.log(wc) + gammaln(beta / gamma)) / gamma
+
+
+def gmw_adm_cwt(gamma=3., beta=60.):
+    """integral(psih(w)^2 / w, 0..inf) (utils/cwt_utils.py:50-63):
+    4 e^(2 wc^g) wc^(-2b) Gamma(2b/g) / (g 2^(2b/g))."""
+    from scipy.special import gammaln
+    wc = (beta / gamma)**(1 / gamma)
+    return 4 * np.exp(2 * wc**gamma - 2 * beta * np.log(wc) + gammaln(2 * beta / gamma)
+                      - (2 * beta / gamma) * np.log(2)) / gamma
+
+
+def issq_full(Tx, scale):
+    """issq_cwt / issq_stft full inversion (_ssq_cwt.py:364-378,
+    _ssq_stft.py:192-198) with NumPy's semantics: the axis-0 sum of Re Tx
+    in Tx's real dtype, then times the float64 scale in place."""
+    x = Tx.real.sum(axis=0)
+    x *= scale
+    return x
+
+
+def invert_components(Tx, cc, cw, scale):
+    """_invert_components (_ssq_cwt.py:381-403), then times scale: rows
+    clip(cc - cw) .. clip(cc + cw) of each column per component (float64
+    sums), the rest (Tx's dtype) last; cc = -1 marks no curve."""
+    rows, N = Tx.shape
+    K = cc.shape[1]
+    x = np.zeros((K + 1, N))
+    rem = Tx.copy()
+    for n in range(K):
+        up = np.clip(cc[:, n] + cw[:, n], 0, rows)
+        lo = np.clip(cc[:, n] - cw[:, n], 0, rows)
+        up[cc[:, n] == -1] = 0
+        lo[cc[:, n] == -1] = 1
+        mask = np.zeros(Tx.shape, dtype='complex128')
+        for m in range(N):
+            sl = slice(lo[m], up[m] + 1)
+            mask[sl, m] = Tx[sl, m]
+            rem[sl, m] = 0
+        x[n] = mask.real.sum(axis=0)
+    x[K] = rem.real.sum(axis=0)
+    x *= scale
+    return x
+
+
+def icwt_1int(Wx, segments):
+    """One-integral icwt (_cwt.py:437-494) with l1_norm: per (r0, r1, k,
+    div) segment, k * sum of Re Wx[r0:r1] / div (div None = norm 1), the
+    segments added."""
+    x = None
+    for r0, r1, k, div in segments:
+        W = Wx[r0:r1].real
+        if div is not None:
+            W = W / div.reshape(-1, 1)
+        xs = W.sum(axis=0) * k
+        x = xs if x is None else x + xs
+    return x

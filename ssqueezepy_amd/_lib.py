@@ -109,6 +109,10 @@ _SIGS = {
                                c_int, c_void_p, c_void_p, c_double, c_void_p,
                                c_void_p, c_void_p, c_void_p,
                                ctypes.POINTER(BinParams), c_double, c_void_p]),
+    'ssqk_real_rowsum': (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64,
+                                 c_void_p, c_double, c_double, c_int, c_void_p, c_void_p]),
+    'ssqk_real_bandsum': (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64,
+                                  c_double, c_void_p, c_void_p]),
 }
 EXPORTED = tuple(_SIGS)
 
@@ -356,3 +360,19 @@ def stft_exec(x, n_fft, hop, pad, modulated, window, diff_window, fs, Sx, dSx,
         _ptr(Sx), _ptr(dSx), _ptr(Sfs), _ptr(bins),
         ctypes.byref(bp) if bp is not None else None, float(gamma),
         _stream(x.device)))
+
+
+def real_rowsum(A, r0, r1, scale, x, div=None, add=0., accumulate=False):
+    """x[b] (float32 [batch, N]) = scale * sum of Re A[b, r0:r1] (see
+    ssqk_real_rowsum)."""
+    batch = 1 if A.ndim == 2 else A.shape[0]
+    check(load().ssqk_real_rowsum(_ptr(A), batch, A.shape[-2], A.shape[-1], int(r0), int(r1),
+                                  _ptr(div), float(scale), float(add), int(bool(accumulate)),
+                                  _ptr(x), _stream(A.device)))
+
+
+def real_bandsum(A, cc, cw, scale, x):
+    """x (float64 [K + 1, N]) = curve-band sums of Re A [rows, N] (see
+    ssqk_real_bandsum); cc, cw int32 [N, K] on the device."""
+    check(load().ssqk_real_bandsum(_ptr(A), A.shape[-2], A.shape[-1], _ptr(cc), _ptr(cw),
+                                   cc.shape[1], float(scale), _ptr(x), _stream(A.device)))

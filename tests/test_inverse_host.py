@@ -1,0 +1,117 @@
+"""Inverses on the host side (no GPU): the admissibility constants against
+closed forms and an independent integrator, and the oracle's inversions
+against the reference's reconstruction thresholds
+(tests/reconstruction_test.py:21-123) on the float64 chain."""
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import ssq_oracle as O
+
+
+def _t(lo, hi, N):
+    return np.linspace(lo, hi, N, endpoint=False)
+
+
+def _cos_f(freqs, N=128):
+    return np.concatenate([np.cos(2 * np.pi * f * _t(i, i + 1, N))
+                           for i, f in enumerate(freqs)])
+
+
+def _freqs(N, freqs):
+    x = _cos_f(freqs, N // len(freqs))
+    return x, _t(0, len(x) / N, len(x))
+
+
+# the reference's reconstruction test signals (reconstruction_test.py:35-56)
+SIGNALS = {
+    'echirp': lambda N: (np.cos(2 * np.pi * 3 * np.exp(_t(0, 10, N) / 3)), _t(0, 10, N)),
+    'lchirp': lambda N: (np.cos(np.pi * _t(0, 10, N)**2), _t(0, 10, N)),
+    'fast_transitions': lambda N: _freqs(N, np.array([N / 100, N / 200, N / 3, N / 20,
+                                                      N / 3 - 1, N / 50, N / 4,
+                                                      N / 150]) / 8),
+    'low_freqs': lambda N: _freqs(N, [.3, .3, 1, 1, 2, 2]),
+    'high_freqs': lambda N: _freqs(N, np.array([N / 2, N / 2 - 1, N / 4, N / 3]) / 4),
+}
+
+
+def mad_rms(x, xrec):
+    return np.mean(np.abs(x - xrec)) / np.sqrt(np.mean(x**2))
+
+
+def test_adm_ssq_and_adm_cwt():
+    """integrate_analytic restated: within its trapezoid error of the GMW
+    closed forms, and of scipy's adaptive quadrature for Morlet."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._inverse import adm_ssq, adm_cwt
+    for beta in (60., 8.):
+        w = sq.Wavelet(('gmw', {'beta': beta}))
+        assert abs(adm_ssq(w) / O.gmw_adm_ssq(3., beta) - 1) < 1e-5
+        assert abs(adm_cwt(w) / O.gmw_adm_cwt(3., beta) - 1) < 1e-5
+    fn = sq.Wavelet(('morlet', {'mu': 6.})).fn
+    q = integrate.quad(lambda v: fn(np.array([v]))[0] / v, 0, 40, limit=400)[0]
+    assert abs(adm_ssq(('morlet', {'mu': 6.})) / q - 1) < 1e-6
+
+
+def _cases():
+    for name in SIGNALS:
+        for scales in ('log', 'log-piecewise', 'linear'):
+            if name == 'low_freqs':
+                if scales == 'linear':
+                    continue
+                scales = scales + ':maximal'
+            yield name, scales
+
+
+@pytest.mark.parametrize('name,scales', list(_cases()))
+def test_oracle_issq_cwt_reconstructs(name, scales):
+    """reconstruction_test.py:65-87: MAD/RMS of issq_cwt(ssq_cwt(x)) < 0.1
+    (GMW beta=8, float64)."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._ssq_cwt import ssq_cwt_grid
+    from ssqueezepy_amd._inverse import adm_ssq
+    x, ts = SIGNALS[name](2048)
+    wav = sq.Wavelet(('gmw', {'beta': 8., 'dtype': 'float64'}))
+    g = ssq_cwt_grid(len(x), wav, scales=scales, nv=32, t=ts)
+    Tx = O.ssq_cwt(x, O.psih_fn('gmw', beta=8.), g['scales64'], g['ssq_freqs'],
+                   g['const'], g['bp'].type != 0, 10 * np.finfo(np.float64).eps,
+                   dt=g['dt'])[0]
+    xrec = O.issq_full(Tx, 2 / adm_ssq(wav))
+    assert mad_rms(x, xrec) < 0.1, (name, scales, mad_rms(x, xrec))
+
+
+def test_oracle_components_add_up_to_the_full_inversion():
+    """Curve bands that do not overlap: components + remainder = the full
+    inversion (float64 against the float32-sum remainder's rounding)."""
+    rng = np.random.default_rng(0)
+    rows, N = 60, 300
+    Tx = (rng.standard_normal((rows, N)) + 1j * rng.standard_normal((rows, N))
+          ).astype(np.complex64)
+    cc = np.stack([rng.integers(5, 20, N), rng.integers(35, 55, N)], axis=1).astype(np.int32)
+    cw = np.stack([rng.integers(0, 5, N), rng.integers(0, 5, N)], axis=1).astype(np.int32)
+    cc[::7, 0] = -1
+    x = O.invert_components(Tx, cc, cw, 1.0)
+    full = Tx.astype(np.complex128).real.sum(axis=0)
+    assert np.allclose(x.sum(axis=0), full, rtol=0, atol=1e-5)
+    # a band of every row is the full inversion, nothing left over
+    allb = O.invert_components(Tx, np.full((N, 1), rows // 2, np.int32),
+                               np.full((N, 1), rows, np.int32), 1.0)
+    assert np.array_equal(allb[0], full) and not allb[1].any()
+
+
+def test_oracle_icwt_reconstructs():
+    """reconstruction_test.py:90-110 (l1_norm=True): MAD/RMS of
+    icwt(cwt(x)) < 0.1, one-integral inverse, log scales."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._inverse import adm_ssq
+    from ssqueezepy_amd._scales import process_scales
+    wav = sq.Wavelet(('gmw', {'beta': 8., 'dtype': 'float64'}))
+    for name in SIGNALS:
+        x, ts = SIGNALS[name](2048)
+        stype = 'log:maximal' if name in ('low_freqs', 'high_freqs') else 'log'
+        scales, _, _, nv = process_scales(stype, len(x), wav, nv=32, get_params=True)
+        s64 = np.asarray(scales, dtype=np.float64).reshape(-1)
+        Wx, _ = O.cwt(x, O.psih_fn('gmw', beta=8.), s64, dt=ts[1] - ts[0])
+        k = (2 / adm_ssq(wav)) * np.log(2 ** (1 / nv))
+        xrec = O.icwt_1int(Wx, [(0, len(s64), k, None)])
+        assert mad_rms(x, xrec) < 0.1, (name, mad_rms(x, xrec))
