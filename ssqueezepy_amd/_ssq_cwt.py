@@ -164,10 +164,12 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
         # otherwise the fused ssqueeze_fast test |Wx| > gamma
         plan = run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx,
                        w=w, bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w)
-        Wsq = _apply_squeezing(Wx, squeezing).contiguous()
+        Wsq, real = _apply_squeezing(Wx, squeezing)
         Tx = torch.empty_like(Wx)
         cst = _device_const(g, dev)
-        _lib.squeeze(Wsq, bins, cst, na, Tx)
+        _lib.squeeze(Wsq.contiguous(), bins, cst, na, Tx)
+        if real:
+            Tx = Tx.real.contiguous()
         extra = tuple(v for v in (w, dWx) if v is not None)
         return Tx, Wx, extra, bins, plan, cst, na
 

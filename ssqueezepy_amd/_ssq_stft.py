@@ -71,10 +71,12 @@ def ssq_stft(x, window=None, n_fft=None, win_len=None, hop_len=1, fs=None, t=Non
     else:
         _lib.stft_exec(x_dev, n_fft, hop_len, padtype, modulated, win, dwin, fs,
                        Sx, dSx, Sfs=Sfs_t, bins=bins, bp=bp, gamma=gamma)
-    Ssq = _apply_squeezing(Sx, squeezing).contiguous()
+    Ssq, real = _apply_squeezing(Sx, squeezing)
     Tx = torch.empty_like(Sx)
     cst = device_constant(np.full(rows, const, dtype=np.float32), torch.float32, dev)
-    _lib.squeeze(Ssq, bins, cst, rows, Tx)
+    _lib.squeeze(Ssq.contiguous(), bins, cst, rows, Tx)
+    if real:
+        Tx = Tx.real.contiguous()
     if flipud:
         ssq_freqs = ssq_freqs[::-1]
     if not get_dWx:

@@ -135,14 +135,23 @@ def cwt_const(scales, cwt_scaletype, nv, na):
 
 
 def _apply_squeezing(Wx, squeezing):
+    """The values squeezed in place of Wx (ssqueezing.py:196-202), as
+    complex64 for the squeeze kernel, and whether Tx is real: 'abs' (and a
+    function returning real values) squeeze a real array, so the reference's
+    Tx (zeros of that dtype) is real.  'lebesgue' divides by len(Wx), the
+    first axis, as the reference does (the batch size of a 3D Wx)."""
     import torch
     if isinstance(squeezing, FunctionType):
-        return squeezing(Wx)
+        v = squeezing(Wx)
+        if not torch.is_tensor(v):
+            v = torch.as_tensor(np.asarray(v), device=Wx.device)
+        real = not v.is_complex()
+        return v.to(device=Wx.device, dtype=Wx.dtype).contiguous(), real
     if squeezing == 'lebesgue':
-        return torch.ones_like(Wx) / Wx.shape[-2]
+        return torch.ones_like(Wx) / len(Wx), False
     if squeezing == 'abs':
-        return torch.abs(Wx).to(Wx.dtype)
-    return Wx
+        return torch.abs(Wx).to(Wx.dtype), True
+    return Wx, False
 
 
 def _check_ssqueezing_args(squeezing, maprange=None, wavelet=None, difftype=None,
@@ -248,10 +257,12 @@ def ssqueeze(Wx, w=None, ssq_freqs=None, scales=None, Sfs=None, fs=None, t=None,
         _lib.bins_from_dwx(Wx, dWx, Sfs_t, bp, gamma, bins)
     else:
         _lib.bins_from_w(w, bp, bins)
-    Wsq = _apply_squeezing(Wx, squeezing).contiguous()
+    Wsq, real = _apply_squeezing(Wx, squeezing)
     Tx = torch.empty_like(Wx)
     cst = device_constant(const, torch.float32, device)
-    _lib.squeeze(Wsq, bins, cst, na, Tx)
+    _lib.squeeze(Wsq.contiguous(), bins, cst, na, Tx)
+    if real:
+        Tx = Tx.real.contiguous()
 
     if (transform == 'cwt' and not flipud) or flipud:
         ssq_freqs = ssq_freqs[::-1]
