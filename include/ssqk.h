@@ -237,6 +237,22 @@ int ssqk_real_bandsum(const void* A, int64_t rows, int64_t N, const int32_t* cc,
                       const int32_t* cw, int64_t K, double scale, double* x,
                       void* stream);
 
+/* ------------------------------------------------------ ridge tracking
+ * Forward-backward ridge tracking (extract_ridges, ridge_extraction.py:
+ * 11-232) on Tf [nf, N] complex64: energy |Tf|^2; per ridge i < n_ridges,
+ * E = -log(energy / max_f energy + eps), the forward accumulated penalty
+ * with jumps penalty * (s_f - s_j)^2 (s: float32 [nf], log scales for a
+ * CWT or frequencies for an STFT), the backward pass of the serial
+ * reference, then energy[r - bw : r + bw] cleared per column (Python slice
+ * bounds).  ridge int32 [n_ridges][N]; ridge_e float32 [n_ridges][N] (the
+ * energy along each ridge before its clearing) may be NULL.  nf <= 13000;
+ * workspace of ssqk_ridge_workspace_size bytes. */
+int ssqk_ridge_workspace_size(int64_t nf, int64_t N, size_t* bytes);
+int ssqk_ridge_track(const void* Tf, int64_t nf, int64_t N, const float* s,
+                     float penalty, float eps, int n_ridges, int bw,
+                     int32_t* ridge, float* ridge_e, void* workspace,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,9 @@ ssqueezepy package root).  How each function is pinned:
   icwt_1int         the reference's reconstruction thresholds
                     (tests/reconstruction_test.py:65-123) on the float64
                     chain; components + remainder = full inversion
+  ridge_track       known answer (tests/ridge_extraction_test.py:17-26);
+                    the tracked path has the least total cost among all
+                    paths (brute force on tiny maps)
 """
 import numpy as np
 
@@ -325,3 +328,47 @@ def icwt_1int(Wx, segments):
         xs = W.sum(axis=0) * k
         x = xs if x is None else x + xs
     return x
+
+
+# ------------------------------------------------------------------ ridges
+def ridge_track(Tf, scales, penalty=2., n_ridges=1, bw=15, transform='cwt', cr=True):
+    """extract_ridges (ridge_extraction.py:11-232), serial path, float32 as
+    the reference evaluates a complex64 map: -log(energy / max + eps),
+    forward accumulated penalty (min over jumps penalty * (s_f - s_j)^2),
+    backward pass keeping the last frequency within eps, then the band
+    energy[r - bw : r + bw] cleared per column.  ``cr``: |Tf| and the log
+    correctly rounded (the GPU's evaluation), else NumPy's own float32
+    ones.  Returns (ridge [N, K], ridge energies [N, K])."""
+    eps = np.float32(np.finfo(np.float32).eps)
+    scales = np.asarray(scales, dtype=np.float32)
+    s = (np.log(scales) if transform == 'cwt' else scales).squeeze()
+    P = np.float32(penalty) * np.subtract.outer(s, s)**2
+    T = np.asarray(Tf, dtype=np.complex64)
+    if cr:
+        # |Tf| and log correctly rounded to float32 (NumPy's SIMD float32
+        # log and complex abs are within 1 ulp of it, platform-dependent)
+        h = np.sqrt(T.real.astype(np.float64)**2 + T.imag.astype(np.float64)**2)
+        energy = h.astype(np.float32)**2
+    else:
+        energy = np.abs(T)**2
+    nf, N = energy.shape
+    ridges = np.zeros((N, n_ridges), dtype=int)
+    ridge_e = np.zeros((N, n_ridges), dtype=np.float32)
+    for i in range(n_ridges):
+        q = energy / energy.max(axis=0) + eps
+        E = -(np.log(q.astype(np.float64)).astype(np.float32) if cr else np.log(q))
+        pen = E.copy()
+        for t in range(1, N):
+            pen[:, t] += np.min(pen[:, t - 1][None, :] + P, axis=1)
+        r = np.argmin(pen, axis=0) % N
+        for t in range(N - 2, -1, -1):
+            rn = r[t + 1]
+            val = pen[rn, t + 1] - E[rn, t + 1]
+            hit = np.nonzero(np.abs(val - (pen[:, t] + P[rn, :])) < eps)[0]
+            if len(hit):
+                r[t] = hit[-1]
+        ridges[:, i] = r
+        ridge_e[:, i] = energy[r, np.arange(N)]
+        for t in range(N):
+            energy[int(r[t] - bw):int(r[t] + bw), t] = 0
+    return ridges, ridge_e

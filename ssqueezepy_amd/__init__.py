@@ -18,6 +18,7 @@ from ._stft import stft, get_window
 from ._ssq_stft import ssq_stft, phase_stft
 from .ssqueezing import ssqueeze
 from ._inverse import issq_cwt, issq_stft, icwt, adm_ssq, adm_cwt
+from .ridge_extraction import extract_ridges
 from . import algos
 
 __version__ = '0.1.0'

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, 'csrc')
 INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 OBJDIR = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libssqk.so')
-SOURCES = ['api.hip', 'cwt.hip', 'stft.hip', 'inverse.hip']
+SOURCES = ['api.hip', 'cwt.hip', 'stft.hip', 'inverse.hip', 'ridge.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=gfx950',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',

@@ -113,6 +113,10 @@ _SIGS = {
                                  c_void_p, c_double, c_double, c_int, c_void_p, c_void_p]),
     'ssqk_real_bandsum': (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64,
                                   c_double, c_void_p, c_void_p]),
+    'ssqk_ridge_workspace_size': (c_int, [c_int64, c_int64, ctypes.POINTER(c_size_t)]),
+    'ssqk_ridge_track': (c_int, [c_void_p, c_int64, c_int64, c_void_p, ctypes.c_float,
+                                 ctypes.c_float, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
 }
 EXPORTED = tuple(_SIGS)
 
@@ -376,3 +380,16 @@ def real_bandsum(A, cc, cw, scale, x):
     ssqk_real_bandsum); cc, cw int32 [N, K] on the device."""
     check(load().ssqk_real_bandsum(_ptr(A), A.shape[-2], A.shape[-1], _ptr(cc), _ptr(cw),
                                    cc.shape[1], float(scale), _ptr(x), _stream(A.device)))
+
+
+def ridge_track(Tf, s, penalty, eps, n_ridges, bw, ridge, ridge_e=None):
+    """ssqk_ridge_track on Tf (complex64 [nf, N]); ridge int32 [n_ridges, N],
+    ridge_e float32 [n_ridges, N] or None."""
+    import torch
+    nf, N = Tf.shape
+    n = c_size_t()
+    check(load().ssqk_ridge_workspace_size(nf, N, ctypes.byref(n)))
+    ws = torch.empty(n.value, dtype=torch.uint8, device=Tf.device)
+    check(load().ssqk_ridge_track(_ptr(Tf), nf, N, _ptr(s), float(penalty), float(eps),
+                                  int(n_ridges), int(bw), _ptr(ridge), _ptr(ridge_e),
+                                  _ptr(ws), _stream(Tf.device)))

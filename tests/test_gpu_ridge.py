@@ -1,0 +1,84 @@
+"""extract_ridges on the GPU (ssqk_ridge_track) against the oracle's
+restatement of the reference's serial path: identical ridge indices and
+energies, on the reference's known-answer map, random maps and the CWT /
+SSQ-CWT / STFT maps of a two-component signal (tests/ridge_extraction_test.py)."""
+import numpy as np
+import pytest
+
+from oracle import ssq_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(Tf, scales, **kw):
+    import ssqueezepy_amd as sq
+    T = Tf.cpu().numpy() if hasattr(Tf, 'cpu') else Tf
+    s = scales.cpu().numpy() if hasattr(scales, 'cpu') else scales
+    r, f, e = sq.extract_ridges(Tf, scales, get_params=True, **kw)
+    kw.pop('parallel', None)
+    ro, eo = O.ridge_track(T, s, **kw)
+    assert r.shape == ro.shape and np.array_equal(r, ro)
+    assert np.array_equal(e, eo)
+    assert np.array_equal(f, np.asarray(s, dtype=np.float32).reshape(-1)[r])
+    return r
+
+
+def test_known_answer():
+    """ridge_extraction_test.py:17-26."""
+    import ssqueezepy_amd as sq
+    print("ridge known answer", flush=True)
+    T = np.array([[1, 4, 4], [2, 2, 2], [5, 5, 4]])
+    r = sq.extract_ridges(T, np.exp([1, 2, 3]), penalty=2.0)
+    assert np.array_equal(r[:, 0], [2, 2, 2])
+
+
+@pytest.mark.parametrize('nf,N,penalty,n_ridges,bw', [(7, 50, 2., 1, 2), (40, 300, 0.5, 3, 4),
+                                                    (300, 700, 2., 2, 15), (1100, 40, 5., 2, 30)])
+def test_random_maps(nf, N, penalty, n_ridges, bw):
+    """Random maps, bands near the edges (negative slice starts), more rows
+    than one block of threads."""
+    rng = np.random.default_rng(nf + N)
+    print(f"ridge random nf={nf} N={N}", flush=True)
+    T = (rng.standard_normal((nf, N)) + 1j * rng.standard_normal((nf, N))).astype(np.complex64)
+    scales = np.geomspace(1, 100, nf)
+    _check(T, scales, penalty=penalty, n_ridges=n_ridges, bw=bw)
+    _check(T, np.linspace(0, .5, nf), penalty=penalty, n_ridges=n_ridges, bw=bw,
+           transform='stft')
+
+
+def test_numpy_float32_evaluation_gives_the_same_ridges():
+    """The reference evaluates |Tf| and log with NumPy's float32 routines,
+    which differ from correct rounding by an ulp on some entries (SIMD
+    dispatch, platform-dependent); the tracked ridges are the same."""
+    import ssqueezepy_amd as sq
+    rng = np.random.default_rng(7)
+    n = np.arange(2000)
+    x = (np.cos(2 * np.pi * (0.02 * n + 0.05 * n**2 / 2000)) + np.cos(2 * np.pi * 0.3 * n)
+         + 0.3 * rng.standard_normal(2000)).astype(np.float32)
+    print("ridge numpy evaluation", flush=True)
+    Wx, scales = sq.cwt(x, 'gmw')
+    r = sq.extract_ridges(Wx, scales, n_ridges=2)
+    rn, _ = O.ridge_track(Wx.cpu().numpy(), scales.cpu().numpy(), n_ridges=2, cr=False)
+    assert np.array_equal(r, rn)
+
+
+def test_transform_maps():
+    """Cubic-polynomial chirp + tone (ridge_extraction_test.py:29-43, wrap
+    padding for the CWT) through cwt, ssq_cwt, stft and ssq_stft, two ridges
+    each."""
+    import scipy.signal as sig
+    import ssqueezepy_amd as sq
+    N = 257
+    t = np.linspace(0, 10, N, endpoint=True)
+    x = sig.sweep_poly(t, np.poly1d([0.025, -0.36, 1.25, 2.0])) + np.sin(2 * np.pi * .5 * t)
+    x = x.astype(np.float32)
+    print("ridge transform maps", flush=True)
+    Twx, Wx, fc, scales = sq.ssq_cwt(x, 'morlet', t=t, padtype='wrap')[:4]
+    # (n_fft = N = 257 there: the GPU path takes powers of two)
+    Tsx, Sx, fs_, Sfs = sq.ssq_stft(x, n_fft=128, fs=1 / (t[1] - t[0]), padtype='zero')[:4]
+    ckw = dict(penalty=.5, n_ridges=2, transform='cwt')
+    skw = dict(penalty=.5, n_ridges=2, transform='stft')
+    _check(Wx, scales, bw=15, **ckw)
+    _check(Twx, fc.copy(), bw=4, **ckw)
+    _check(Sx, Sfs, bw=15, **skw)
+    _check(Tsx, fs_.copy(), bw=4, **skw)
