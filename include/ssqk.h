@@ -245,7 +245,7 @@ int ssqk_real_bandsum(const void* A, int64_t rows, int64_t N, const int32_t* cc,
  * CWT or frequencies for an STFT), the backward pass of the serial
  * reference, then energy[r - bw : r + bw] cleared per column (Python slice
  * bounds).  ridge int32 [n_ridges][N]; ridge_e float32 [n_ridges][N] (the
- * energy along each ridge before its clearing) may be NULL.  nf <= 13000;
+ * energy along each ridge before its clearing) may be NULL.  nf <= 10000;
  * workspace of ssqk_ridge_workspace_size bytes. */
 int ssqk_ridge_workspace_size(int64_t nf, int64_t N, size_t* bytes);
 int ssqk_ridge_track(const void* Tf, int64_t nf, int64_t N, const float* s,

@@ -58,31 +58,63 @@ __global__ __launch_bounds__(kNT) void k_ridge_E(const float* __restrict__ energ
 }
 
 // pen[:, 0] = E[:, 0]; pen[f][t] = E[f][t] + min_j (pen[j][t-1] + jump(s_f, s_j))
-// (np.amin: a NaN anywhere gives NaN)
+// (np.amin: a NaN anywhere gives NaN).  tpf threads share a frequency f,
+// each taking jc consecutive j (a multiple of 4: float4 reads of the
+// previous column, broadcast to the wave) with four running minima; the
+// minimum of a set does not depend on the order it is taken in, so the
+// split is exact.  LDS: s and two columns padded to nfp (+inf past nf,
+// which never wins), and the partial minima [tpf][nf].
 __global__ void k_ridge_fw(const float* __restrict__ E, const float* __restrict__ s, int nf,
-                           int64_t N, float penalty, float* __restrict__ pen) {
-  extern __shared__ float sh[];              // s [nf], prev [2][nf]
+                           int64_t N, float penalty, int tpf, int jc,
+                           float* __restrict__ pen) {
+  extern __shared__ float sh[];
+  const int nfp = (nf + 3) & ~3;
   float* ss = sh;
-  float* prev = sh + nf;
+  float* prev = sh + nfp;
+  float* part = sh + 3 * nfp;
   const int nt = blockDim.x;
-  for (int f = threadIdx.x; f < nf; f += nt) {
-    ss[f] = s[f];
-    const float e0 = E[(int64_t)f * N];
-    prev[f] = e0;
-    pen[(int64_t)f * N] = e0;
+  for (int e = threadIdx.x; e < nfp; e += nt) {
+    ss[e] = e < nf ? s[e] : 0.f;
+    const float e0 = e < nf ? E[(int64_t)e * N] : INFINITY;
+    prev[e] = e0;
+    prev[nfp + e] = INFINITY;
+    if (e < nf) pen[(int64_t)e * N] = e0;
   }
   __syncthreads();
   for (int64_t t = 1; t < N; ++t) {
-    const float* pv = prev + ((t - 1) & 1) * nf;
-    float* nx = prev + (t & 1) * nf;
-    for (int f = threadIdx.x; f < nf; f += nt) {
+    const float* pv = prev + ((t - 1) & 1) * nfp;
+    float* nx = prev + (t & 1) * nfp;
+    for (int w = threadIdx.x; w < tpf * nf; w += nt) {
+      const int c = w / nf, f = w - c * nf;
       const float sf = ss[f];
-      float m = pv[0] + jump(penalty, sf, ss[0]);
-      for (int j = 1; j < nf; ++j) {
-        const float v = pv[j] + jump(penalty, sf, ss[j]);
-        m = (v < m || isnan(v)) && !isnan(m) ? v : m;
+      float m0 = INFINITY, m1 = INFINITY, m2 = INFINITY, m3 = INFINITY;
+      bool nan = false;
+      const int j0 = c * jc, j1 = min(j0 + jc, nfp);
+      for (int j = j0; j < j1; j += 4) {
+        const float4 p4 = *reinterpret_cast<const float4*>(pv + j);
+        const float4 s4 = *reinterpret_cast<const float4*>(ss + j);
+        const float v0 = p4.x + jump(penalty, sf, s4.x);
+        const float v1 = p4.y + jump(penalty, sf, s4.y);
+        const float v2 = p4.z + jump(penalty, sf, s4.z);
+        const float v3 = p4.w + jump(penalty, sf, s4.w);
+        nan = nan || isnan(v0) || isnan(v1) || isnan(v2) || isnan(v3);
+        m0 = fminf(m0, v0);
+        m1 = fminf(m1, v1);
+        m2 = fminf(m2, v2);
+        m3 = fminf(m3, v3);
       }
-      const float p = E[(int64_t)f * N + t] + m;
+      part[c * nf + f] = nan ? NAN : fminf(fminf(m0, m1), fminf(m2, m3));
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < nf; f += nt) {
+      float m = part[f];
+      bool nan = isnan(m);
+      for (int c = 1; c < tpf; ++c) {
+        const float q = part[c * nf + f];
+        nan = nan || isnan(q);
+        m = fminf(m, q);
+      }
+      const float p = E[(int64_t)f * N + t] + (nan ? NAN : m);
       nx[f] = p;
       pen[(int64_t)f * N + t] = p;
     }
@@ -175,8 +207,9 @@ extern "C" int ssqk_ridge_track(const void* Tf, int64_t nf, int64_t N, const flo
                                 int32_t* ridge, float* ridge_e, void* workspace,
                                 void* stream) {
   if (!Tf || !s || !ridge || !workspace || nf < 1 || N < 1 || n_ridges < 1 ||
-      nf > 13000 || nf * N >= ((int64_t)1 << 40) || N >= ((int64_t)1 << 31)) {
-    // (nf <= 13000: s and two columns of the recursion, 12 nf bytes, in LDS)
+      nf > 10000 || nf * N >= ((int64_t)1 << 40) || N >= ((int64_t)1 << 31)) {
+    // (nf <= 10000: s, two columns and the partial minima of the
+    // recursion, 16 (nf + 3) bytes at most, in 160 KB of LDS)
     ssqk_host::set_error("bad arguments to ssqk_ridge_track");
     return SSQK_ERR_ARG;
   }
@@ -185,7 +218,17 @@ extern "C" int ssqk_ridge_track(const void* Tf, int64_t nf, int64_t N, const flo
   float* E = energy + nf * N;
   float* pen = E + nf * N;
   const int nt = (int)std::min<int64_t>(1024, (nf + 63) / 64 * 64);
-  const int lds = (int)(3 * nf * sizeof(float));
+  // forward recursion: up to 8 threads per frequency, j in chunks of a
+  // multiple of 4
+  const int nfp = (int)((nf + 3) & ~3);
+  const int tpf = (int)std::max<int64_t>(1, std::min<int64_t>(8, 1024 / nf));
+  const int jc = ((nfp + tpf - 1) / tpf + 3) & ~3;
+  const int nt_fw = (int)std::min<int64_t>(1024, (tpf * nf + 63) / 64 * 64);
+  const int lds = (int)((3 * nfp + tpf * nf) * sizeof(float));
+  if (lds > 160 * 1024) {
+    ssqk_host::set_error("ssqk_ridge_track: nf=%lld too large", (long long)nf);
+    return SSQK_ERR_ARG;
+  }
   int rc = ssqk_host::allow_dynamic_lds((const void*)k_ridge_fw, lds);
   if (rc) return rc;
   hipLaunchKernelGGL(k_abs2, dim3(blocks(nf * N)), dim3(kNT), 0, st, (const float2*)Tf,
@@ -195,7 +238,8 @@ extern "C" int ssqk_ridge_track(const void* Tf, int64_t nf, int64_t N, const flo
     int32_t* ri = ridge + (int64_t)i * N;
     hipLaunchKernelGGL(k_ridge_E, dim3(blocks(N)), dim3(kNT), 0, st, energy, (int)nf, N, eps, E);
     SSQK_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_ridge_fw, dim3(1), dim3(nt), lds, st, E, s, (int)nf, N, penalty, pen);
+    hipLaunchKernelGGL(k_ridge_fw, dim3(1), dim3(nt_fw), lds, st, E, s, (int)nf, N, penalty,
+                       tpf, jc, pen);
     SSQK_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_ridge_argmin, dim3(blocks(N)), dim3(kNT), 0, st, pen, (int)nf, N, ri);
     SSQK_CHECK(hipGetLastError());
