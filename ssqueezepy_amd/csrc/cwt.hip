@@ -1038,6 +1038,7 @@ struct ssqk_cwt_plan_s {
     int ntasks = 0;
     int nscales = 0;
     int Hmin = 0, Hmax = 0;
+    double work = 0;          // sum over groups of ns * N * (log2 F + 3) * F / B
     OsGroup* d_groups = nullptr;
     int2* d_tasks = nullptr;
   };
@@ -1476,6 +1477,7 @@ int plan_create_impl(ssqk_cwt_plan_s** out, int64_t N, int64_t L, int64_t n1, in
       for (size_t gi = 0; gi < grp_by_F[c].size(); ++gi) {
         const OsGroup& g = grp_by_F[c][gi];
         oc.nscales += g.ns;
+        oc.work += g.ns * (double)N * (std::log2((double)oc.F) + 3) * oc.F / (double)g.B;
         oc.Hmin = std::min(oc.Hmin, g.H);
         oc.Hmax = std::max(oc.Hmax, g.H);
         const int64_t nblk = (N + g.B - 1) / g.B;
@@ -1761,18 +1763,17 @@ int exec_impl(ssqk_cwt_plan_s* p, const float* x, int64_t batch, void* Wx, void*
     double cost;
   };
   std::vector<Item> items;
-  const double N = (double)p->N, L = (double)p->L;
-  for (size_t c = 0; c < p->os.size(); ++c) {
-    const auto& o = p->os[c];
-    items.push_back({0, (int)c, o.nscales * N * (std::log2((double)o.F) + 3) * o.F /
-                                    (double)std::max(o.F - 2 * o.Hmax, 1)});
-  }
+  const double L = (double)p->L;
+  // estimated cost per item, fitted to serial kernel times: overlap-save
+  // per group by its own block B, the four-step path at log2 L + 15
+  for (size_t c = 0; c < p->os.size(); ++c)
+    items.push_back({0, (int)c, p->os[c].work});
   for (size_t c = 0; c < p->classes.size(); ++c) {
     const auto& k = p->classes[c];
     items.push_back({1, (int)c, k.nscales * L * (std::log2((double)k.M) + 3)});
   }
   if (!p->chunks.empty())
-    items.push_back({2, 0, (double)p->full.size() * L * (std::log2(L) + 8)});
+    items.push_back({2, 0, (double)p->full.size() * L * (std::log2(L) + 15)});
   std::sort(items.begin(), items.end(),
             [](const Item& u, const Item& v) { return u.cost > v.cost; });
   constexpr int NS = ssqk_cwt_plan_s::kSide + 1;
