@@ -115,3 +115,13 @@ def test_oracle_icwt_reconstructs():
         k = (2 / adm_ssq(wav)) * np.log(2 ** (1 / nv))
         xrec = O.icwt_1int(Wx, [(0, len(s64), k, None)])
         assert mad_rms(x, xrec) < 0.1, (name, mad_rms(x, xrec))
+
+
+def test_adm_coefficients_stay_away_from_zero():
+    """adm_coef_test.py:17-36: the numeric integrals of 200 Morlet
+    wavelets (mu = 4 .. 30) and of GMW beta = 2 .. 120 stay above 1e-3
+    (an unstable integration gives values near zero)."""
+    from ssqueezepy_amd._inverse import adm_ssq, adm_cwt
+    for wav in ([('morlet', {'mu': mu}) for mu in np.linspace(4, 30, 200)] +
+                [('gmw', {'beta': b}) for b in np.linspace(2, 120, 40)]):
+        assert adm_cwt(wav) > 1e-3 and adm_ssq(wav) > 1e-3, wav
