@@ -214,6 +214,16 @@ int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t n_fft,
                    const float* Sfs, uint16_t* bins, const ssqk_bin_params* bp,
                    double gamma, void* stream);
 
+/* Frequency-domain derivative of complex rows (complex64): A [rows, N_in]
+ * padded to L (a power of two <= 8192) with pad_type at left pad n1, or,
+ * with pad_type < 0, A already [rows, L]; FFT, times i xi fs (xi the scale-1
+ * frequency grid, +pi at the Nyquist bin), inverse FFT, out [rows, N_out]
+ * the positions [n1, n1 + N_out).  Replaces trigdiff (utils/common.py:
+ * 161-245), the dWx of a given Wx in phase_transform (experimental.py:
+ * 201-213). */
+int ssqk_trigdiff(const void* A, int64_t rows, int64_t N_in, int64_t N_out, int64_t L,
+                  int64_t n1, int pad_type, double fs, void* out, void* stream);
+
 /* ------------------------------------------------------------- inverses
  * Column sums of the real part of A [batch, rows, N] complex64, rows
  * r0 <= i < r1 in ascending order in float32 (NumPy's axis-0 order):

@@ -113,6 +113,8 @@ _SIGS = {
                                  c_void_p, c_double, c_double, c_int, c_void_p, c_void_p]),
     'ssqk_real_bandsum': (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64,
                                   c_double, c_void_p, c_void_p]),
+    'ssqk_trigdiff': (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int,
+                              c_double, c_void_p, c_void_p]),
     'ssqk_ridge_workspace_size': (c_int, [c_int64, c_int64, ctypes.POINTER(c_size_t)]),
     'ssqk_ridge_track': (c_int, [c_void_p, c_int64, c_int64, c_void_p, ctypes.c_float,
                                  ctypes.c_float, c_int, c_int, c_void_p, c_void_p, c_void_p,
@@ -393,3 +395,12 @@ def ridge_track(Tf, s, penalty, eps, n_ridges, bw, ridge, ridge_e=None):
     check(load().ssqk_ridge_track(_ptr(Tf), nf, N, _ptr(s), float(penalty), float(eps),
                                   int(n_ridges), int(bw), _ptr(ridge), _ptr(ridge_e),
                                   _ptr(ws), _stream(Tf.device)))
+
+
+def trigdiff(A, N_out, L, n1, pad, fs, out):
+    """ssqk_trigdiff: A complex64 [rows, N_in] (pad a PAD_CODES name) or
+    [rows, L] (pad None); out complex64 [rows, N_out]."""
+    rows = int(np.prod(A.shape[:-1]))
+    check(load().ssqk_trigdiff(_ptr(A), rows, A.shape[-1], int(N_out), int(L), int(n1),
+                               PAD_CODES[pad] if pad is not None else -1, float(fs),
+                               _ptr(out), _stream(A.device)))

@@ -768,6 +768,50 @@ __global__ __launch_bounds__(fx_threads(L1)) void k_fwd_B(FwdArgs f,
   }
 }
 
+// ----------------------------------------------------------- trigdiff
+// Frequency-domain derivative of complex rows (utils/common.py:161-245):
+// row r of A padded to L (pad >= 0: A rows of length Nin through pad_src
+// with left pad n1; pad < 0: A rows already of length L), FFT, times
+// i xi_k fs / L (xi the scale-1 grid, +pi at the Nyquist bin), inverse FFT,
+// positions [n1, n1 + Nout) out.  Rows as in the band path:
+// max(2048 / L, 1) per workgroup.
+template <int L>
+__global__ __launch_bounds__(band_threads(L)) void k_trigdiff(const float2* __restrict__ A,
+                                                               int64_t rows, int Nin, int Nout,
+                                                               int n1, int pad, float fs,
+                                                               float2* __restrict__ out,
+                                                               const float2* __restrict__ tw) {
+  constexpr int ROWS = band_rows(L);
+  constexpr int NT = band_threads(L);
+  __shared__ float2 s[Lds<L>::elems(ROWS)];
+  const int64_t r0 = (int64_t)blockIdx.x * ROWS;
+  const int64_t in_len = pad >= 0 ? Nin : L;
+  for (int e = threadIdx.x; e < ROWS * L; e += NT) {
+    const int r = e / L, p = e - r * L;
+    float2 v = make_float2(0.f, 0.f);
+    if (r0 + r < rows) {
+      const int64_t src = pad >= 0 ? pad_src(p, Nin, n1, pad) : p;
+      if (src >= 0) v = A[(r0 + r) * in_len + src];
+    }
+    s[Lds<L>::idx(r, p)] = v;
+  }
+  float2* const bufs[1] = {s};
+  lds_fft<L, ROWS, -1, NT, 1>(bufs, tw);
+  const float c = (float)(kTwoPi / L) * fs / (float)L;
+  for (int e = threadIdx.x; e < ROWS * L; e += NT) {
+    const int r = e / L, k = e - r * L;
+    const int kk = k <= L / 2 ? k : k - L;
+    const float q = (float)kk * c;             // xi_k fs / L
+    const float2 z = s[Lds<L>::idx(r, k)];
+    s[Lds<L>::idx(r, k)] = make_float2(-q * z.y, q * z.x);
+  }
+  lds_fft<L, ROWS, +1, NT, 1>(bufs, tw);
+  for (int e = threadIdx.x; e < ROWS * Nout; e += NT) {
+    const int r = e / Nout, j = e - r * Nout;
+    if (r0 + r < rows) out[(r0 + r) * Nout + j] = s[Lds<L>::idx(r, n1 + j)];
+  }
+}
+
 // ------------------------------------------------------------ dispatch
 #define SSQK_POW2_CASES(MACRO) \
   MACRO(2) MACRO(4) MACRO(8) MACRO(16) MACRO(32) MACRO(64) MACRO(128) MACRO(256) \
@@ -1638,6 +1682,34 @@ extern "C" int ssqk_cwt_plan_create(ssqk_cwt_plan* out, int64_t N, int64_t L, in
                                     int64_t na, double dt, double band_rel) {
   return ssqk_cwt_plan_create2(out, N, L, n1, pad_type, wavelet_kind, wavelet_params, scales,
                                na, dt, band_rel, nullptr);
+}
+
+extern "C" int ssqk_trigdiff(const void* A, int64_t rows, int64_t N_in, int64_t N_out,
+                             int64_t L, int64_t n1, int pad_type, double fs, void* out,
+                             void* stream) {
+  if (!A || !out || rows < 1 || N_out < 1 || L < 2 || L > kP || (L & (L - 1)) != 0 ||
+      n1 < 0 || n1 + N_out > L || (pad_type >= 0 && (pad_type > SSQK_PAD_WRAP || N_in < 1)) ||
+      rows >= ((int64_t)1 << 31)) {
+    ssqk_host::set_error("bad arguments to ssqk_trigdiff (L=%lld must be a power of two "
+                         "<= %d holding [n1, n1 + N_out))", (long long)L, kP);
+    return SSQK_ERR_ARG;
+  }
+  SSQK_GET_TW(tw, (int)L);
+  hipStream_t st = (hipStream_t)stream;
+  switch ((int)L) {
+#define C(m)                                                                           \
+  case m:                                                                              \
+    hipLaunchKernelGGL(k_trigdiff<m>, dim3((unsigned)((rows + band_rows(m) - 1) / band_rows(m))), \
+                       dim3(band_threads(m)), 0, st, (const float2*)A, rows, (int)N_in,     \
+                       (int)N_out, (int)n1, pad_type, (float)fs, (float2*)out, tw);     \
+    break;
+    SSQK_POW2_CASES(C)
+#undef C
+    default:
+      return SSQK_ERR_UNSUPPORTED;
+  }
+  SSQK_CHECK(hipGetLastError());
+  return SSQK_OK;
 }
 
 extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {

@@ -17,15 +17,12 @@ def phase_transform(Wx, dWx=None, difftype='trig', difforder=4, gamma=None, fs=1
                     Sfs=None, rpadded=False, padtype='reflect', N=None, n1=None,
                     get_w=False, transform='cwt'):
     """(w, Wx, dWx, Sfs, gamma) of experimental.py:192-262: w by phase_cwt /
-    phase_stft when ``get_w``; dWx must be given (the trigonometric
-    derivative of a user Wx, trigdiff, is not built)."""
+    phase_stft when ``get_w``; a missing dWx of a CWT is trigdiff of Wx."""
     from ._ssq_cwt import phase_cwt
     from ._ssq_stft import phase_stft, _make_Sfs
     if transform == 'stft' and dWx is None:
         raise NotImplementedError("`phase_transform` without `dWx` for "
                                   "STFT is not currently supported.")
-    if dWx is None:
-        raise NotImplementedError("`phase_transform` without `dWx` (trigdiff) is not built")
     if rpadded and N is None:
         raise ValueError("`rpadded=True` requires `N`")
     if getattr(Wx, 'ndim', np.ndim(Wx)) > 2 and get_w:
@@ -37,6 +34,15 @@ def phase_transform(Wx, dWx=None, difftype='trig', difforder=4, gamma=None, fs=1
         gamma = 10 * (EPS64 if is64 else EPS32)
     w = None
     if transform == 'cwt':
+        if dWx is None:
+            # experimental.py:201-209: dWx by trigdiff of the given Wx
+            from .utils import trigdiff
+            if N is None and not rpadded:
+                N = Wx.shape[-1]
+            if n1 is None:
+                from ._common import p2up
+                n1 = p2up(N)[1]
+            dWx = trigdiff(Wx, fs, padtype, rpadded, N=N, n1=n1, transform='cwt')
         if get_w:
             w = phase_cwt(Wx, dWx, difftype, gamma)
     else:
