@@ -26,6 +26,9 @@ extern "C" {
 
 #define SSQK_WAVELET_GMW 0
 #define SSQK_WAVELET_MORLET 1
+/* GMW of order k (_gmw.py:267-394); params: gamma, beta, k + 1, then the
+ * k + 1 Laguerre constants of _gmw_k_constants (bandpass), k <= 15 */
+#define SSQK_WAVELET_GMW_K 2
 
 #define SSQK_PAD_REFLECT 0
 #define SSQK_PAD_ZERO 1

@@ -24,6 +24,9 @@ ssqueezepy package root).  How each function is pinned:
   ssq_cwt, ssq_stft golden e2e fixtures (Tx, Wx, w, dWx) and C1 samples
   gmw_adm_ssq,
   gmw_adm_cwt       closed forms (Gamma-function integrals of the GMW)
+  gmw_k_consts,
+  gmw_k_psih        order 0 is gmw_psih; orders are mutually orthogonal
+                    (Laguerre polynomials under the weight x^c e^-x)
   issq_full,
   invert_components,
   icwt_1int         the reference's reconstruction thresholds
@@ -91,7 +94,38 @@ def morlet_psih(w, mu=13.4):
     return C1 * (np.exp(-.5 * (w - mu)**2) - ks * np.exp(-.5 * w**2))
 
 
+def gmw_k_consts(gamma=3., beta=60., k=1):
+    """_gmw.py:366-394 (bandpass): 2 * coeff * (-1)^m Gamma(k+c+1) /
+    (Gamma(c+m+1) Gamma(k-m+1) m!), r = (2 beta + 1) / gamma, c = r - 1,
+    coeff = sqrt(Gamma(r) k! / Gamma(k + r))."""
+    from scipy.special import gammaln
+    r = (2 * beta + 1) / gamma
+    c = r - 1
+    coeff = np.sqrt(np.exp(gammaln(r) + gammaln(k + 1) - gammaln(k + r)))
+    m = np.arange(k + 1)
+    L = (-1.)**m * np.exp(gammaln(k + c + 1) - gammaln(c + m + 1) - gammaln(k - m + 1)
+                          - gammaln(m + 1))
+    return 2 * coeff * L
+
+
+def gmw_k_psih(w, gamma=3., beta=60., k=1):
+    """_gmw.py:267-293: the order-k bandpass GMW, sum_m kc[m] (2 w^gamma)^m
+    times exp(-beta ln wc + wc^gamma + beta ln w - w^gamma), 0 for w < 0."""
+    w = np.asarray(w, dtype=np.float64)
+    kc = gmw_k_consts(gamma, beta, k)
+    wc = (beta / gamma)**(1 / gamma)
+    out = np.zeros_like(w)
+    m = w > 0
+    u = 2 * w[m]**gamma
+    C = np.polyval(kc[::-1], u)
+    out[m] = C * np.exp(-beta * np.log(wc) + wc**gamma + beta * np.log(w[m]) - w[m]**gamma)
+    return out
+
+
 def psih_fn(kind, **params):
+    if kind == 'gmw_k':
+        return lambda w: gmw_k_psih(w, params.get('gamma', 3.), params.get('beta', 60.),
+                                    params.get('k', 1))
     if kind == 'gmw':
         return lambda w: gmw_psih(w, params.get('gamma', 3.), params.get('beta', 60.))
     return lambda w: morlet_psih(w, params.get('mu', 13.4))

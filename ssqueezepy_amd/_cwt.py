@@ -68,15 +68,56 @@ def _maybe_autotune(device, N, L, n1, pad, wavelet, scales64, dt, padtype):
         _autotune.autotune(N, wavelet, padtype=padtype)
 
 
+KIND_GMW_K = 2   # SSQK_WAVELET_GMW_K
+
+
+def gmw_order_params(wavelet, k):
+    """(kind, parameters) of the order-k bandpass GMW with ``wavelet``'s
+    gamma and beta (_gmw.py:267-293): gamma, beta, k + 1 and the Laguerre
+    constants of _gmw_k_constants (_gmw.py:366-394), the factor 2 included,
+    in float64."""
+    from scipy.special import gammaln, gamma as gamma_fn
+    p = wavelet.fn.params64
+    g, b = float(p['gamma']), float(p['beta'])
+    k = int(k)
+    if k < 0 or k > 15:
+        raise ValueError("GMW order must be in 0..15 (got %s)" % k)
+    r = (2 * b + 1) / g
+    c = r - 1
+    coeff = np.sqrt(np.exp(gammaln(r) + gammaln(k + 1) - gammaln(k + r)))
+    consts = np.zeros(k + 1)
+    for m in range(k + 1):
+        fact = np.exp(gammaln(k + c + 1) - gammaln(c + m + 1) - gammaln(k - m + 1))
+        consts[m] = (-1)**m * fact / gamma_fn(m + 1)
+    consts = consts * coeff * 2
+    return KIND_GMW_K, (g, b, float(k + 1)) + tuple(float(v) for v in consts)
+
+
+def gmw_orders(wavelet, order, average):
+    """Orders and averaging of cwt_higher_order (_cwt.py:551-577)."""
+    if wavelet.kind != KIND_GMW:
+        raise ValueError("`wavelet` must be GMW for higher-order transforms "
+                         "(got %s)" % wavelet.name)
+    if isinstance(order, (list, range)):
+        order = tuple(order)
+    orders = list(order) if isinstance(order, tuple) else [order]
+    if len(orders) == 1 and average:
+        average = False
+    avg = bool(average or (average is None and isinstance(order, tuple)))
+    return orders, avg, isinstance(order, tuple)
+
+
 def run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=None, w=None,
-            bins=None, bp=None, gamma=0., strict=True):
-    """Launch the CWT plan for device signal(s) ``x_dev`` (float32 [B,] N)."""
+            bins=None, bp=None, gamma=0., strict=True, kind_params=None):
+    """Launch the CWT plan for device signal(s) ``x_dev`` (float32 [B,] N);
+    ``kind_params`` = (kind, parameters) overrides the wavelet's own (a
+    higher-order GMW)."""
     N = x_dev.shape[-1]
     L, n1, pad = _padding(N, padtype)
-    if autotune_env():
+    if kind_params is None and autotune_env():
         _maybe_autotune(x_dev.device, N, L, n1, pad, wavelet, scales64, dt, padtype)
-    plan = _lib.cwt_plan(x_dev.device, N, L, n1, pad, wavelet.kind,
-                         _wavelet_kernel_params(wavelet), scales64, dt)
+    kind, prm = kind_params or (wavelet.kind, _wavelet_kernel_params(wavelet))
+    plan = _lib.cwt_plan(x_dev.device, N, L, n1, pad, kind, prm, scales64, dt)
     plan.exec(x_dev, Wx, dWx=dWx, w=w, bins=bins, bp=bp, gamma=gamma,
               strict=strict)
     return plan
@@ -92,8 +133,7 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
     Returns (Wx, scales) or (Wx, scales, dWx); Wx [na, N] or [B, na, N]
     complex64 on the GPU (NumPy with astensor=False)."""
     import torch
-    if isinstance(order, (tuple, list, range)) or order > 0:
-        raise NotImplementedError("higher-order GMW CWT is not on the GPU path")
+    higher = isinstance(order, (tuple, list, range)) or order > 0
     if not l1_norm:
         raise NotImplementedError("l1_norm=False is not on the GPU path")
     if rpadded:
@@ -121,6 +161,34 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
         dWx = torch.empty_like(Wx) if derivative else None
         plan = run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=dWx)
         return Wx, ((dWx,) if derivative else ()), plan
+
+    if higher:
+        # cwt_higher_order (_cwt.py:517-610): one transform per GMW order on
+        # the zeroth order's scales, averaged (or listed)
+        if grad:
+            raise NotImplementedError("gradients of higher-order GMW CWTs are not built")
+        orders, avg, listed = gmw_orders(wavelet, order, average)
+        x_dev = signal_to_device(x, wavelet.dtype)
+        shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
+        Ws, dWs = [], []
+        for k in orders:
+            Wk = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
+            dWk = torch.empty_like(Wk) if derivative else None
+            run_cwt(x_dev, wavelet, scales64, dt, padtype, Wk, dWx=dWk,
+                    kind_params=gmw_order_params(wavelet, k))
+            Ws.append(Wk)
+            dWs.append(dWk)
+        if avg:
+            Ws = [torch.stack(Ws).mean(0)]
+            dWs = [torch.stack(dWs).mean(0) if derivative else None]
+        scales_out = device_constant(scales64.astype(wavelet.dtype), torch.float32,
+                                     x_dev.device, copy=True)
+        Ws = [to_output(v, astensor) for v in Ws]
+        dWs = [to_output(v, astensor) for v in dWs]
+        scales_out = to_output(scales_out, astensor)
+        Wx_all = Ws if (listed and not avg) else Ws[0]
+        dWx_all = dWs if (listed and not avg) else dWs[0]
+        return (Wx_all, scales_out, dWx_all) if derivative else (Wx_all, scales_out)
 
     if grad:
         x_dev = differentiable_input(x, wavelet.dtype)

@@ -193,7 +193,7 @@ class CwtPlan:
         self.N, self.L, self.n1, self.na = int(N), int(L), int(n1), len(scales64)
         self.opts = opts
         handle = c_void_p()
-        p = (c_double * 2)(*params)
+        p = (c_double * len(params))(*params)
         s = np.ascontiguousarray(scales64, dtype=np.float64)
         with torch.cuda.device(device):
             check(lib.ssqk_cwt_plan_create2(

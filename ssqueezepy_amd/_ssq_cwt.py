@@ -138,8 +138,7 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
     import torch
     if x.ndim == 2 and get_w:
         raise NotImplementedError("`get_w=True` unsupported with batched input.")
-    if isinstance(order, (tuple, list, range)) or order > 0:
-        raise NotImplementedError("higher-order GMW ssq_cwt is not on the GPU path")
+    higher = isinstance(order, (tuple, list, range)) or order > 0
     difforder = _check_ssqueezing_args(squeezing, maprange, wavelet, difftype,
                                        difforder, get_w, transform='cwt')
     grad = wants_grad(x)
@@ -160,10 +159,37 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
         dWx = torch.empty_like(Wx) if get_dWx else None
         w = torch.empty(shape, dtype=torch.float32, device=dev) if get_w else None
         bins = torch.empty(shape, dtype=torch.int16, device=dev)
-        # get_w: phase_cwt then indexed_sum_onfly (w = inf where |Wx| < gamma);
-        # otherwise the fused ssqueeze_fast test |Wx| > gamma
-        plan = run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx,
-                       w=w, bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w)
+        plan = None
+        if higher:
+            # _ssq_cwt.py:227-243: Wx averaged over the GMW orders on the
+            # zeroth order's scales, its derivative (the reference
+            # differentiates the padded average with trigdiff; the average
+            # of each order's frequency-domain derivative is the same
+            # linear map), then the phase transform and squeeze of ssq_cwt
+            from ._cwt import gmw_order_params, gmw_orders
+            orders, _, _ = gmw_orders(wavelet, order, True)
+            Ws, dWs = [], []
+            for k in orders:
+                Wk = torch.empty_like(Wx)
+                dWk = torch.empty_like(Wx)
+                run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wk, dWx=dWk,
+                        kind_params=gmw_order_params(wavelet, k))
+                Ws.append(Wk)
+                dWs.append(dWk)
+            Wx = torch.stack(Ws).mean(0) if len(Ws) > 1 else Ws[0]
+            dW = torch.stack(dWs).mean(0) if len(dWs) > 1 else dWs[0]
+            if get_dWx:
+                dWx = dW
+            if get_w:
+                _lib.phase(Wx, dW, None, gamma, w)
+                _lib.bins_from_w(w, g['bp'], bins)
+            else:
+                _lib.bins_from_dwx(Wx, dW, None, g['bp'], gamma, bins)
+        else:
+            # get_w: phase_cwt then indexed_sum_onfly (w = inf where |Wx| <
+            # gamma); otherwise the fused ssqueeze_fast test |Wx| > gamma
+            plan = run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx,
+                           w=w, bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w)
         Wsq, real = _apply_squeezing(Wx, squeezing)
         Tx = torch.empty_like(Wx)
         cst = _device_const(g, dev)
@@ -176,6 +202,8 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
     if grad:
         if squeezing != 'sum':
             raise NotImplementedError("gradients need squeezing='sum' (Tx linear in Wx)")
+        if higher:
+            raise NotImplementedError("gradients of higher-order GMW ssq_cwt are not built")
         x_dev = differentiable_input(x, wavelet.dtype)
         outs = SsqCwtFunction.apply(x_dev, run)
         Tx, Wx = outs[0], outs[1]

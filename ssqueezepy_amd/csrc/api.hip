@@ -355,23 +355,18 @@ extern "C" int ssqk_psih(int wavelet_kind, const double* params, const double* s
     ssqk_host::set_error("bad arguments to ssqk_psih");
     return SSQK_ERR_ARG;
   }
-  WavDev wav{};
-  if (wavelet_kind == SSQK_WAVELET_GMW) {
-    const double g = params[0], b = params[1];
-    if (!(g > 0) || !(b > 0)) {
-      ssqk_host::set_error("GMW needs gamma > 0 and beta > 0");
-      return SSQK_ERR_ARG;
+  WavDev wav;
+  {
+    const int rc = make_wav_dev(wavelet_kind, params, &wav, nullptr);
+    if (rc == SSQK_ERR_ARG) {
+      ssqk_host::set_error("GMW needs gamma > 0, beta > 0 (and 1..%d order constants)",
+                           kMaxGmwOrder + 1);
+      return rc;
     }
-    const double wc = std::pow(b / g, 1.0 / g);
-    wav = WavDev{SSQK_WAVELET_GMW, g, b, std::log(wc), std::pow(wc, g)};
-  } else if (wavelet_kind == SSQK_WAVELET_MORLET) {
-    const double mu = params[0];
-    const double cs = std::pow(1 + std::exp(-mu * mu) - 2 * std::exp(-0.75 * mu * mu), -0.5);
-    wav = WavDev{SSQK_WAVELET_MORLET, mu, std::exp(-0.5 * mu * mu), -0.5,
-                 std::sqrt(2.0) * cs * std::pow(M_PI, 0.25)};
-  } else {
-    ssqk_host::set_error("unknown wavelet kind %d", wavelet_kind);
-    return SSQK_ERR_UNSUPPORTED;
+    if (rc) {
+      ssqk_host::set_error("unknown wavelet kind %d", wavelet_kind);
+      return rc;
+    }
   }
   hipStream_t st = (hipStream_t)stream;
   const int gx = (int)std::min<int64_t>((L + 255) / 256, 1024);

@@ -11,21 +11,66 @@ constexpr float kTwoPiF = 6.283185307179586f;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 // Wavelet parameters in float64.  GMW (kind 0): a=gamma, b=beta, c=ln wc,
-// d=wc^gamma.  Morlet (kind 1): a=mu, b=ks, c=C0 (-1/2), d=C1.
+// d=wc^gamma.  Morlet (kind 1): a=mu, b=ks, c=C0 (-1/2), d=C1.  GMW of
+// order k (kind 2): as GMW, with the k + 1 Laguerre constants kc (the
+// bandpass factor 2 included, _gmw.py:366-394).
+constexpr int kMaxGmwOrder = 15;
 struct WavDev {
   int kind;
   double a, b, c, d;
+  int nk;
+  double kc[kMaxGmwOrder + 1];
 };
 
-// psih(w) in float64 (_gmw.py:203-210, wavelets.py:521-523).
-__device__ __forceinline__ double psih_eval(const WavDev& p, double w) {
-  if (p.kind == SSQK_WAVELET_GMW) {
+// psih(w) in float64 (_gmw.py:203-210, 267-293, wavelets.py:521-523).
+__host__ __device__ inline double psih_value64(const WavDev& p, double w) {
+  if (p.kind == SSQK_WAVELET_GMW || p.kind == SSQK_WAVELET_GMW_K) {
     if (!(w > 0.0)) return 0.0;
     const double lw = log(w);
-    return 2.0 * exp(p.b * (lw - p.c) - (exp(p.a * lw) - p.d));
+    const double e = exp(p.b * (lw - p.c) - (exp(p.a * lw) - p.d));
+    if (p.kind == SSQK_WAVELET_GMW) return 2.0 * e;
+    // sum_m kc[m] (2 w^gamma)^m by Horner
+    const double u = 2.0 * exp(p.a * lw);
+    double C = p.kc[p.nk - 1];
+    for (int m = p.nk - 2; m >= 0; --m) C = C * u + p.kc[m];
+    return C * e;
   }
   const double u = w - p.a;
   return p.d * (exp(p.c * u * u) - p.b * exp(p.c * w * w));
+}
+__device__ __forceinline__ double psih_eval(const WavDev& p, double w) {
+  return psih_value64(p, w);
+}
+
+// Device parameters of wavelet `kind` from its ABI parameter list (GMW:
+// gamma, beta; Morlet: mu; GMW of order k: gamma, beta, k + 1 and the
+// constants); the centre frequency of the order-0 wavelet.  0 or an error.
+inline int make_wav_dev(int kind, const double* prm, WavDev* w, double* center) {
+  *w = WavDev{};
+  w->kind = kind;
+  if (kind == SSQK_WAVELET_GMW || kind == SSQK_WAVELET_GMW_K) {
+    const double g = prm[0], b = prm[1];
+    if (!(g > 0) || !(b > 0)) return SSQK_ERR_ARG;
+    const double wc = pow(b / g, 1.0 / g);
+    w->a = g; w->b = b; w->c = log(wc); w->d = pow(wc, g);
+    if (center) *center = wc;
+    if (kind == SSQK_WAVELET_GMW_K) {
+      const int n = (int)prm[2];
+      if (n < 1 || n > kMaxGmwOrder + 1) return SSQK_ERR_ARG;
+      w->nk = n;
+      for (int m = 0; m < n; ++m) w->kc[m] = prm[3 + m];
+    }
+    return SSQK_OK;
+  }
+  if (kind == SSQK_WAVELET_MORLET) {
+    const double mu = prm[0];
+    const double cs = pow(1 + exp(-mu * mu) - 2 * exp(-0.75 * mu * mu), -0.5);
+    w->a = mu; w->b = exp(-0.5 * mu * mu); w->c = -0.5;
+    w->d = sqrt(2.0) * cs * pow(3.14159265358979323846, 0.25);
+    if (center) *center = mu > 0 ? mu : 1.0;
+    return SSQK_OK;
+  }
+  return SSQK_ERR_UNSUPPORTED;
 }
 
 // One signal of a ragged batch (signals of different lengths that share a

@@ -962,18 +962,39 @@ int launch_fwd(const FwdArgs& f, int batch, hipStream_t st, int L1, int L2, floa
 }
 
 // ----------------------------------------------------------- host math
-double psih_host(const WavDev& p, double w) {
-  if (p.kind == SSQK_WAVELET_GMW) {
-    if (!(w > 0.0)) return 0.0;
-    const double lw = std::log(w);
-    return 2.0 * std::exp(p.b * (lw - p.c) - (std::exp(p.a * lw) - p.d));
-  }
-  const double u = w - p.a;
-  return p.d * (std::exp(p.c * u * u) - p.b * std::exp(p.c * w * w));
-}
+double psih_host(const WavDev& p, double w) { return psih_value64(p, w); }
 
 // [w_lo, w_hi]: |psih| >= rel * |psih(center)| inside (unimodal wavelets)
 void band_limits(const WavDev& p, double center, double rel, double* lo, double* hi) {
+  if (p.kind == SSQK_WAVELET_GMW_K) {
+    // a higher-order GMW crosses zero between its lobes: scan a log grid
+    // for the outermost points above rel * max |psih|, then bisect the
+    // bracketing steps
+    const int n = 20000;
+    const double l0 = std::log(1e-12), l1 = std::log(center * 64 + 64);
+    std::vector<double> g(n), v(n);
+    double mx = 0;
+    for (int i = 0; i < n; ++i) {
+      g[i] = std::exp(l0 + (l1 - l0) * i / (n - 1));
+      v[i] = std::fabs(psih_host(p, g[i]));
+      mx = std::max(mx, v[i]);
+    }
+    const double th = rel * mx;
+    int i0 = 0, i1 = n - 1;
+    while (i0 < n - 1 && v[i0] < th) ++i0;
+    while (i1 > 0 && v[i1] < th) --i1;
+    auto edge = [&](double a, double b, bool rising) {
+      for (int it = 0; it < 200; ++it) {
+        const double m = 0.5 * (a + b);
+        const bool in = std::fabs(psih_host(p, m)) >= th;
+        if (in == rising) b = m; else a = m;
+      }
+      return rising ? a : b;
+    };
+    *lo = i0 == 0 ? 0.0 : edge(g[i0 - 1], g[i0], true) * (1 - 1e-9);
+    *hi = i1 == n - 1 ? g[n - 1] : edge(g[i1], g[i1 + 1], false) * (1 + 1e-9);
+    return;
+  }
   const double th = rel * std::fabs(psih_host(p, center));
   double a = 1e-12, b = center;
   if (std::fabs(psih_host(p, a)) >= th) {
@@ -1634,26 +1655,19 @@ extern "C" int ssqk_cwt_plan_create2(ssqk_cwt_plan* out, int64_t N, int64_t L, i
     ssqk_host::set_error("unknown pad type %d", pad_type);
     return SSQK_ERR_ARG;
   }
-  WavDev wav{};
-  double center;
-  if (wavelet_kind == SSQK_WAVELET_GMW) {
-    const double gamma = wavelet_params[0], beta = wavelet_params[1];
-    if (!(gamma > 0) || !(beta > 0)) {
-      ssqk_host::set_error("GMW needs gamma > 0 and beta > 0");
-      return SSQK_ERR_ARG;
+  WavDev wav;
+  double center = 1.0;
+  {
+    const int rc = make_wav_dev(wavelet_kind, wavelet_params, &wav, &center);
+    if (rc == SSQK_ERR_ARG) {
+      ssqk_host::set_error("GMW needs gamma > 0, beta > 0 (and 1..%d order constants)",
+                           kMaxGmwOrder + 1);
+      return rc;
     }
-    const double wc = std::pow(beta / gamma, 1.0 / gamma);
-    wav = WavDev{SSQK_WAVELET_GMW, gamma, beta, std::log(wc), std::pow(wc, gamma)};
-    center = wc;
-  } else if (wavelet_kind == SSQK_WAVELET_MORLET) {
-    const double mu = wavelet_params[0];
-    const double cs = std::pow(1 + std::exp(-mu * mu) - 2 * std::exp(-0.75 * mu * mu), -0.5);
-    const double ks = std::exp(-0.5 * mu * mu);
-    wav = WavDev{SSQK_WAVELET_MORLET, mu, ks, -0.5, std::sqrt(2.0) * cs * std::pow(M_PI, 0.25)};
-    center = mu > 0 ? mu : 1.0;
-  } else {
-    ssqk_host::set_error("unknown wavelet kind %d", wavelet_kind);
-    return SSQK_ERR_UNSUPPORTED;
+    if (rc) {
+      ssqk_host::set_error("unknown wavelet kind %d", wavelet_kind);
+      return rc;
+    }
   }
   ssqk_cwt_plan_s* p = nullptr;
   const char* env = std::getenv("SSQK_DISABLE_OVERLAP_SAVE");
