@@ -21,7 +21,8 @@ class SsqCwtFunction(torch.autograd.Function):
     def forward(ctx, x, run):
         Tx, Wx, extra, bins, plan, cst, nf = run(x.detach())
         ctx.set_materialize_grads(False)        # unused outputs: None, not zeros
-        ctx.plan, ctx.cst, ctx.nf = plan, cst, nf
+        ctx.plans = plan if isinstance(plan, list) else [plan]
+        ctx.cst, ctx.nf = cst, nf
         ctx.x_shape = x.shape
         ctx.save_for_backward(bins)
         for t in extra:
@@ -39,10 +40,9 @@ class SsqCwtFunction(torch.autograd.Function):
         if gTx is None and gWx is None:
             return None, None
         ref = gTx if gTx is not None else gWx
-        gx = torch.empty(ctx.x_shape, dtype=torch.float32, device=ref.device)
-        ctx.plan.adjoint(gx, gTx=gTx, gWx=gWx,
-                         bins=bins if gTx is not None else None,
-                         cst=ctx.cst if gTx is not None else None, nf=ctx.nf)
+        gx = _adjoint_sum(ctx.plans, ctx.x_shape, ref.device, gTx=gTx, gWx=gWx,
+                          bins=bins if gTx is not None else None,
+                          cst=ctx.cst if gTx is not None else None, nf=ctx.nf)
         return gx, None
 
 
@@ -54,7 +54,7 @@ class CwtFunction(torch.autograd.Function):
     def forward(ctx, x, run):
         Wx, extra, plan = run(x.detach())
         ctx.set_materialize_grads(False)
-        ctx.plan = plan
+        ctx.plans = plan if isinstance(plan, list) else [plan]
         ctx.x_shape = x.shape
         for t in extra:
             ctx.mark_non_differentiable(t)
@@ -67,9 +67,22 @@ class CwtFunction(torch.autograd.Function):
         if gWx is None:
             return None, None
         gWx = gWx.to(torch.complex64).contiguous()
-        gx = torch.empty(ctx.x_shape, dtype=torch.float32, device=gWx.device)
-        ctx.plan.adjoint(gx, gWx=gWx)
-        return gx, None
+        return _adjoint_sum(ctx.plans, ctx.x_shape, gWx.device, gWx=gWx), None
+
+
+def _adjoint_sum(plans, x_shape, device, **kw):
+    """x-gradient of a forward that is the mean of the plans' transforms
+    (one plan: the plain transform; several: GMW orders averaged): the mean
+    of their adjoints."""
+    gx = torch.empty(x_shape, dtype=torch.float32, device=device)
+    plans[0].adjoint(gx, **kw)
+    if len(plans) > 1:
+        part = torch.empty_like(gx)
+        for p in plans[1:]:
+            p.adjoint(part, **kw)
+            gx += part
+        gx /= len(plans)
+    return gx
 
 
 def wants_grad(x):

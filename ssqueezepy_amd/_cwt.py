@@ -164,28 +164,45 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
 
     if higher:
         # cwt_higher_order (_cwt.py:517-610): one transform per GMW order on
-        # the zeroth order's scales, averaged (or listed)
-        if grad:
-            raise NotImplementedError("gradients of higher-order GMW CWTs are not built")
+        # the zeroth order's scales, averaged (or listed); the gradient of an
+        # average is the average of the orders' adjoints
         orders, avg, listed = gmw_orders(wavelet, order, average)
-        x_dev = signal_to_device(x, wavelet.dtype)
-        shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
+
+        def run_orders(ks):
+            def run_h(x_dev):
+                shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
+                Ws, dWs, plans = [], [], []
+                for k in ks:
+                    Wk = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
+                    dWk = torch.empty_like(Wk) if derivative else None
+                    plans.append(run_cwt(x_dev, wavelet, scales64, dt, padtype, Wk, dWx=dWk,
+                                         kind_params=gmw_order_params(wavelet, k)))
+                    Ws.append(Wk)
+                    dWs.append(dWk)
+                W = torch.stack(Ws).mean(0) if len(Ws) > 1 else Ws[0]
+                dW = None
+                if derivative:
+                    dW = torch.stack(dWs).mean(0) if len(dWs) > 1 else dWs[0]
+                return W, ((dW,) if derivative else ()), plans
+            return run_h
+
+        groups = [[k] for k in orders] if (listed and not avg) else [orders]
+        x_dev = (differentiable_input(x, wavelet.dtype) if grad
+                 else signal_to_device(x, wavelet.dtype))
         Ws, dWs = [], []
-        for k in orders:
-            Wk = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
-            dWk = torch.empty_like(Wk) if derivative else None
-            run_cwt(x_dev, wavelet, scales64, dt, padtype, Wk, dWx=dWk,
-                    kind_params=gmw_order_params(wavelet, k))
-            Ws.append(Wk)
-            dWs.append(dWk)
-        if avg:
-            Ws = [torch.stack(Ws).mean(0)]
-            dWs = [torch.stack(dWs).mean(0) if derivative else None]
-        scales_out = device_constant(scales64.astype(wavelet.dtype), torch.float32,
-                                     x_dev.device, copy=True)
+        for ks in groups:
+            if grad:
+                o = CwtFunction.apply(x_dev, run_orders(ks))
+                Ws.append(o[0])
+                dWs.append(o[1] if derivative else None)
+            else:
+                W, extra, _ = run_orders(ks)(x_dev)
+                Ws.append(W)
+                dWs.append(extra[0] if derivative else None)
+        scales_out = to_output(device_constant(scales64.astype(wavelet.dtype), torch.float32,
+                                               x_dev.device, copy=True), astensor)
         Ws = [to_output(v, astensor) for v in Ws]
         dWs = [to_output(v, astensor) for v in dWs]
-        scales_out = to_output(scales_out, astensor)
         Wx_all = Ws if (listed and not avg) else Ws[0]
         dWx_all = dWs if (listed and not avg) else dWs[0]
         return (Wx_all, scales_out, dWx_all) if derivative else (Wx_all, scales_out)

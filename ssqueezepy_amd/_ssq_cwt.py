@@ -168,12 +168,12 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
             # linear map), then the phase transform and squeeze of ssq_cwt
             from ._cwt import gmw_order_params, gmw_orders
             orders, _, _ = gmw_orders(wavelet, order, True)
-            Ws, dWs = [], []
+            Ws, dWs, plan = [], [], []
             for k in orders:
                 Wk = torch.empty_like(Wx)
                 dWk = torch.empty_like(Wx)
-                run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wk, dWx=dWk,
-                        kind_params=gmw_order_params(wavelet, k))
+                plan.append(run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wk,
+                                    dWx=dWk, kind_params=gmw_order_params(wavelet, k)))
                 Ws.append(Wk)
                 dWs.append(dWk)
             Wx = torch.stack(Ws).mean(0) if len(Ws) > 1 else Ws[0]
@@ -202,8 +202,6 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
     if grad:
         if squeezing != 'sum':
             raise NotImplementedError("gradients need squeezing='sum' (Tx linear in Wx)")
-        if higher:
-            raise NotImplementedError("gradients of higher-order GMW ssq_cwt are not built")
         x_dev = differentiable_input(x, wavelet.dtype)
         outs = SsqCwtFunction.apply(x_dev, run)
         Tx, Wx = outs[0], outs[1]
