@@ -10,7 +10,7 @@ from .wavelets import Wavelet, center_frequency, morlet, gmw
 from ._scales import (process_scales, cwt_scalebounds, make_scales,
                       infer_scaletype, logscale_transition_idx, nv_from_scales)
 from ._common import p2up, padsignal_np as padsignal
-from ._cwt import cwt
+from ._cwt import cwt, cwt_higher_order
 from ._ssq_cwt import ssq_cwt, phase_cwt
 from ._batch import ssq_cwt_batch
 from ._autotune import autotune
@@ -19,6 +19,7 @@ from ._ssq_stft import ssq_stft, phase_stft
 from .ssqueezing import ssqueeze
 from ._inverse import issq_cwt, issq_stft, icwt, adm_ssq, adm_cwt
 from .ridge_extraction import extract_ridges
+from . import utils, experimental, configs
 from . import algos
 
 __version__ = '0.1.0'

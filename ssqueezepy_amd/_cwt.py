@@ -219,3 +219,10 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
                                  x_dev.device, copy=True)
     Wx, scales_out, dWx = [to_output(g, astensor) for g in (Wx, scales_out, dWx)]
     return (Wx, scales_out, dWx) if derivative else (Wx, scales_out)
+
+
+def cwt_higher_order(x, wavelet='gmw', order=1, average=None, astensor=True, **kw):
+    """cwt with GMWs of the given order(s) (_cwt.py:517-610): one transform
+    per order on the zeroth order's scales, their mean with ``average``
+    (default for a tuple of orders), else a list."""
+    return cwt(x, wavelet, order=order, average=average, astensor=astensor, **kw)
