@@ -29,6 +29,12 @@ extern "C" {
 /* GMW of order k (_gmw.py:267-394); params: gamma, beta, k + 1, then the
  * k + 1 Laguerre constants of _gmw_k_constants (bandpass), k <= 15 */
 #define SSQK_WAVELET_GMW_K 2
+/* OR'ed into the wavelet kind of ssqk_cwt_plan_create / _create2: the
+ * L2-normalized transform of cwt(l1_norm=False), Wx and dWx times
+ * sqrt(scale) (_cwt.py:307-311), by psih(s xi) sqrt(s) in the plan's
+ * wavelet tables (the adjoint uses the same tables).  ssqk_psih rejects
+ * it. */
+#define SSQK_WAVELET_NORM_L2 0x100
 
 #define SSQK_PAD_REFLECT 0
 #define SSQK_PAD_ZERO 1

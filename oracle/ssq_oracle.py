@@ -15,7 +15,9 @@ ssqueezepy package root).  How each function is pinned:
   gmw_psih,
   morlet_psih       closed forms (_gmw.py:203-210, wavelets.py:521-523),
                     tests: peak value 2 at wc, L1 time norm, golden e2e
-  cwt               brute-force DFT convolution at tiny N; golden e2e Wx/dWx
+  cwt               brute-force DFT convolution at tiny N; golden e2e Wx/dWx;
+                    l1_norm=False: equal time-domain energy of every scale's
+                    impulse response, and the reconstruction threshold
   phase_transform   closed form Im(dWx/Wx)/2pi (tests/fft_test.py:159-161)
   squeeze_bins,
   ssqueeze          brute-force per-point loop; golden squeeze.npz
@@ -132,9 +134,11 @@ def psih_fn(kind, **params):
 
 
 # -------------------------------------------------------------------- cwt
-def cwt(x, psih, scales, dt=1., padtype='reflect', derivative=True):
-    """_cwt.py:167-177, 246-320 (vectorized, l1_norm, float64): pad, fft,
-    psih(scale*xi) with the Nyquist bin halved, ifft, unpad.  x 1D or 2D."""
+def cwt(x, psih, scales, dt=1., padtype='reflect', derivative=True, l1_norm=True):
+    """_cwt.py:167-177, 246-320 (vectorized, float64): pad, fft,
+    psih(scale*xi) with the Nyquist bin halved, ifft, unpad; with
+    l1_norm=False Wx and dWx times sqrt(scale) (_cwt.py:307-311).  x 1D or
+    2D."""
     x = np.asarray(x, dtype=np.float64)
     N = x.shape[-1]
     xp, L, n1, _ = pad_signal(x, padtype)
@@ -145,6 +149,8 @@ def cwt(x, psih, scales, dt=1., padtype='reflect', derivative=True):
     P[:, L // 2] /= 2                                  # wavelets.py:86-95
     if x.ndim == 2:
         xh = xh[:, None]
+    if not l1_norm:
+        P = P * np.sqrt(scales)
     Wx = np.fft.ifft(P * xh, axis=-1)[..., n1:n1 + N]
     if not derivative:
         return Wx, None

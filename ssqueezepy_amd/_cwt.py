@@ -25,6 +25,26 @@ def _wavelet_kernel_params(wavelet):
     return (p['mu'], 0.)
 
 
+KIND_NORM_L2 = 0x100   # SSQK_WAVELET_NORM_L2
+
+
+def process_gmw_wavelet(wavelet, l1_norm):
+    """GMW's norm follows ``l1_norm`` (_cwt.py:497-514): 'bandpass' for
+    the L1 transform, 'energy' for the L2 one (the reference computes that
+    GMW in float64, which is not on this float32 path, so it raises); a
+    GMW L1 ``Wavelet`` with ``l1_norm=False`` is an error."""
+    norm = 'bandpass' if l1_norm else 'energy'
+    if isinstance(wavelet, str) and wavelet.lower()[:3] == 'gmw':
+        return ('gmw', {'norm': norm})
+    if isinstance(wavelet, tuple) and wavelet[0].lower()[:3] == 'gmw':
+        opts = dict(wavelet[1]) if len(wavelet) > 1 else {}
+        opts.setdefault('norm', norm)
+        return (wavelet[0], opts)
+    if isinstance(wavelet, Wavelet) and wavelet.name == 'GMW L1' and not l1_norm:
+        raise ValueError("using GMW L1 wavelet with `l1_norm=False`")
+    return wavelet
+
+
 def _nan_checks(x, nan_checks):
     if nan_checks is None:
         nan_checks = bool(isinstance(x, np.ndarray))
@@ -108,15 +128,17 @@ def gmw_orders(wavelet, order, average):
 
 
 def run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=None, w=None,
-            bins=None, bp=None, gamma=0., strict=True, kind_params=None):
+            bins=None, bp=None, gamma=0., strict=True, kind_params=None, l2=False):
     """Launch the CWT plan for device signal(s) ``x_dev`` (float32 [B,] N);
     ``kind_params`` = (kind, parameters) overrides the wavelet's own (a
-    higher-order GMW)."""
+    higher-order GMW); ``l2``: Wx and dWx times sqrt(scale)."""
     N = x_dev.shape[-1]
     L, n1, pad = _padding(N, padtype)
     if kind_params is None and autotune_env():
         _maybe_autotune(x_dev.device, N, L, n1, pad, wavelet, scales64, dt, padtype)
     kind, prm = kind_params or (wavelet.kind, _wavelet_kernel_params(wavelet))
+    if l2:
+        kind |= KIND_NORM_L2
     plan = _lib.cwt_plan(x_dev.device, N, L, n1, pad, kind, prm, scales64, dt)
     plan.exec(x_dev, Wx, dWx=dWx, w=w, bins=bins, bp=bp, gamma=gamma,
               strict=strict)
@@ -134,8 +156,6 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
     complex64 on the GPU (NumPy with astensor=False)."""
     import torch
     higher = isinstance(order, (tuple, list, range)) or order > 0
-    if not l1_norm:
-        raise NotImplementedError("l1_norm=False is not on the GPU path")
     if rpadded:
         raise NotImplementedError("rpadded=True is not on the GPU path")
     if not hasattr(x, 'ndim'):
@@ -150,7 +170,9 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
         nv = None
     N = x.shape[-1]
     dt, *_ = _process_fs_and_t(fs, t, N=N)
-    wavelet = Wavelet._init_if_not_isinstance(wavelet)
+    # l1_norm=False: GMW takes the 'energy' norm; other wavelets scale Wx
+    # and dWx by sqrt(scale) (_cwt.py:250, 307-311)
+    wavelet = Wavelet._init_if_not_isinstance(process_gmw_wavelet(wavelet, l1_norm))
     scales = process_scales(scales, N, wavelet, nv=nv)
     scales64 = np.asarray(scales, dtype=np.float64).ravel()
     na = len(scales64)
@@ -159,7 +181,8 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
         shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
         Wx = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
         dWx = torch.empty_like(Wx) if derivative else None
-        plan = run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=dWx)
+        plan = run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=dWx,
+                       l2=not l1_norm)
         return Wx, ((dWx,) if derivative else ()), plan
 
     if higher:

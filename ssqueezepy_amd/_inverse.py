@@ -163,15 +163,13 @@ def icwt(Wx, wavelet='gmw', scales='log-piecewise', nv=None, one_int=True, x_len
     from ._scales import process_scales, logscale_transition_idx
     if not one_int:
         raise NotImplementedError("icwt(one_int=False) (double integral) is not built")
-    if not l1_norm:
-        raise NotImplementedError("icwt(l1_norm=False) is not supported on the GPU")
     W = _device_tensor(Wx, 'Wx')
     na, n = W.shape[-2], W.shape[-1]
     x_len = x_len or n
     if not (isinstance(scales, np.ndarray) or torch.is_tensor(scales)) and nv is None:
         nv = 32
-    # (l1_norm: GMW's default 'bandpass' norm, as _process_gmw_wavelet sets it)
-    wavelet = Wavelet._init_if_not_isinstance(wavelet)
+    from ._cwt import process_gmw_wavelet
+    wavelet = Wavelet._init_if_not_isinstance(process_gmw_wavelet(wavelet, l1_norm))
     scales, scaletype, _, nv = process_scales(scales, x_len, wavelet, nv=nv,
                                               get_params=True)
     scales = np.asarray(scales)
@@ -182,14 +180,17 @@ def icwt(Wx, wavelet='gmw', scales='log-piecewise', nv=None, one_int=True, x_len
     def segment(r0, r1, s, accumulate):
         # (the reference's recursion per log segment re-derives its scale
         # type and nv from the segment's scales)
+        # norm(scale) of _icwt_norm (_cwt.py:485-494): L1 1 | scale, L2
+        # scale**.5 | scale**1.5 (log | linear), in float32 as the reference
         _, stype, _, nv_s = process_scales(s, x_len, wavelet, get_params=True)
+        s32 = np.asarray(s, dtype=np.float32).reshape(-1)
         if stype == 'log':
-            div = None                               # norm = 1
+            norm = None if l1_norm else s32**.5
             k = (2 / Css) * np.log(2 ** (1 / nv_s))
         else:
-            div = torch.as_tensor(np.asarray(s, dtype=np.float32).reshape(-1),
-                                  device=W.device)   # norm = scale
+            norm = s32 if l1_norm else s32**1.5
             k = (2 / Css) * np.pi / 4
+        div = None if norm is None else torch.as_tensor(norm, device=W.device)
         _lib.real_rowsum(W, r0, r1, k, x, div=div, add=x_mean, accumulate=accumulate)
 
     if scaletype == 'log-piecewise':

@@ -358,6 +358,10 @@ extern "C" int ssqk_psih(int wavelet_kind, const double* params, const double* s
   WavDev wav;
   {
     const int rc = make_wav_dev(wavelet_kind, params, &wav, nullptr);
+    if (rc == SSQK_OK && wav.l2) {
+      ssqk_host::set_error("ssqk_psih: SSQK_WAVELET_NORM_L2 applies to CWT plans only");
+      return SSQK_ERR_ARG;
+    }
     if (rc == SSQK_ERR_ARG) {
       ssqk_host::set_error("GMW needs gamma > 0, beta > 0 (and 1..%d order constants)",
                            kMaxGmwOrder + 1);

@@ -202,11 +202,13 @@ __device__ __forceinline__ void emit(const CwtArgs& a, const OutRow& r, int n, f
 // v = xh[k] was loaded by the caller (loads are kept out of branches so that
 // they are issued together).  valid = false gives zeros.
 // psih(s xi_k) rounded to float32 with the Nyquist bin halved
-// (wavelets.py:86-95); the plan tabulates it once per scale.
+// (wavelets.py:86-95), times sqrt(s) for the L2 norm (_cwt.py:307-311);
+// the plan tabulates it once per scale.
 __device__ __forceinline__ float psih_value(const WavDev& w, double s, int k, int L) {
   const double xi = (double)k * (kTwoPi / (double)L);
   double ps = psih_eval(w, s * xi);
   if (k == (L >> 1)) ps *= 0.5;
+  if (w.l2) ps *= sqrt(s);
   return (float)ps;
 }
 

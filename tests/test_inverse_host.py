@@ -125,3 +125,64 @@ def test_adm_coefficients_stay_away_from_zero():
     for wav in ([('morlet', {'mu': mu}) for mu in np.linspace(4, 30, 200)] +
                 [('gmw', {'beta': b}) for b in np.linspace(2, 120, 40)]):
         assert adm_cwt(wav) > 1e-3 and adm_ssq(wav) > 1e-3, wav
+
+
+def test_oracle_l2_cwt_equal_energy_per_scale():
+    """cwt(l1_norm=False) (_cwt.py:307-311): the L2-normalized wavelet of
+    every scale has the same time-domain energy, so an impulse's transform
+    has equal row energies wherever the scale's band and support fit the
+    grid; the L1 rows fall as 1/scale."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._scales import process_scales
+    N = 4096
+    x = np.zeros(N)
+    x[N // 2] = 1
+    wav = sq.Wavelet(('morlet', {'mu': 6., 'dtype': 'float64'}))
+    s = np.asarray(process_scales('log', N, wav, nv=16), dtype=np.float64).ravel()
+    mid = slice(len(s) // 4, 3 * len(s) // 4)
+    e = {}
+    for l1 in (True, False):
+        Wx, _ = O.cwt(x, O.psih_fn('morlet', mu=6.), s, padtype='zero', derivative=False,
+                      l1_norm=l1)
+        e[l1] = (np.abs(Wx[mid])**2).sum(-1)
+    assert e[False].max() / e[False].min() - 1 < 1e-9
+    assert np.allclose(e[True] * s[mid], e[True][0] * s[mid][0], rtol=1e-9)
+
+
+def test_oracle_icwt_l2_reconstructs():
+    """reconstruction_test.py:88-107 with l1_norm=False (Morlet: the
+    reference's L2 GMW is float64-only): MAD/RMS of icwt(cwt(x)) < 0.1 with
+    norm(scale) = scale**.5 on log scales."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._inverse import adm_ssq
+    from ssqueezepy_amd._scales import process_scales
+    wav = sq.Wavelet(('morlet', {'mu': 6., 'dtype': 'float64'}))
+    for name in SIGNALS:
+        x, ts = SIGNALS[name](2048)
+        stype = 'log:maximal' if name in ('low_freqs', 'high_freqs') else 'log'
+        scales, _, _, nv = process_scales(stype, len(x), wav, nv=32, get_params=True)
+        s64 = np.asarray(scales, dtype=np.float64).reshape(-1)
+        Wx, _ = O.cwt(x, O.psih_fn('morlet', mu=6.), s64, dt=ts[1] - ts[0], derivative=False,
+                      l1_norm=False)
+        k = (2 / adm_ssq(wav)) * np.log(2 ** (1 / nv))
+        xrec = O.icwt_1int(Wx, [(0, len(s64), k, np.asarray(scales, np.float32).ravel()**.5)])
+        assert mad_rms(x, xrec) < 0.1, (name, mad_rms(x, xrec))
+        Wx1, _ = O.cwt(x, O.psih_fn('morlet', mu=6.), s64, dt=ts[1] - ts[0],
+                       derivative=False)
+        assert np.allclose(Wx, Wx1 * np.sqrt(s64)[:, None], rtol=1e-12, atol=1e-14)
+
+
+def test_l1_norm_false_wavelet_rules():
+    """_process_gmw_wavelet (_cwt.py:497-514): GMW takes the 'energy' norm
+    (float64 in the reference; not on the float32 path), a GMW L1 Wavelet
+    with l1_norm=False is a ValueError, Morlet passes through."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._cwt import process_gmw_wavelet
+    assert process_gmw_wavelet('gmw', False) == ('gmw', {'norm': 'energy'})
+    assert process_gmw_wavelet(('gmw', {'beta': 8}), False) == (
+        'gmw', {'beta': 8, 'norm': 'energy'})
+    assert process_gmw_wavelet('morlet', False) == 'morlet'
+    with pytest.raises(ValueError):
+        process_gmw_wavelet(sq.Wavelet('gmw'), False)
+    with pytest.raises(NotImplementedError):
+        sq.cwt(np.zeros(64, np.float32), 'gmw', l1_norm=False)
