@@ -223,6 +223,15 @@ int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t n_fft,
                    const float* Sfs, uint16_t* bins, const ssqk_bin_params* bp,
                    double gamma, void* stream);
 
+/* Pad float32 rows x [rows, N] to out [rows, L], x at [n1, n1 + N), the
+ * rest by pad_type (numpy pad modes).  Replaces padsignal
+ * (utils/common.py:54-158) on the GPU, the padded input of
+ * cwt(rpadded=True) (_cwt.py:167-177, 293-305), whose Wx is a plan of
+ * length L with no padding (N = L, n1 = 0) run on this padded signal.
+ * rows <= 65535. */
+int ssqk_pad_signal(const float* x, int64_t rows, int64_t N, int64_t L, int64_t n1,
+                    int pad_type, float* out, void* stream);
+
 /* Frequency-domain derivative of complex rows (complex64): A [rows, N_in]
  * padded to L (a power of two <= 8192) with pad_type at left pad n1, or,
  * with pad_type < 0, A already [rows, L]; FFT, times i xi fs (xi the scale-1

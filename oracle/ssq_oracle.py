@@ -17,7 +17,9 @@ ssqueezepy package root).  How each function is pinned:
                     tests: peak value 2 at wc, L1 time norm, golden e2e
   cwt               brute-force DFT convolution at tiny N; golden e2e Wx/dWx;
                     l1_norm=False: equal time-domain energy of every scale's
-                    impulse response, and the reconstruction threshold
+                    impulse response, and the reconstruction threshold;
+                    rpadded: brute-force circular convolution of the
+                    padded signal, unpadded columns = the regular cwt
   phase_transform   closed form Im(dWx/Wx)/2pi (tests/fft_test.py:159-161)
   squeeze_bins,
   ssqueeze          brute-force per-point loop; golden squeeze.npz
@@ -134,11 +136,12 @@ def psih_fn(kind, **params):
 
 
 # -------------------------------------------------------------------- cwt
-def cwt(x, psih, scales, dt=1., padtype='reflect', derivative=True, l1_norm=True):
+def cwt(x, psih, scales, dt=1., padtype='reflect', derivative=True, l1_norm=True,
+        rpadded=False):
     """_cwt.py:167-177, 246-320 (vectorized, float64): pad, fft,
-    psih(scale*xi) with the Nyquist bin halved, ifft, unpad; with
-    l1_norm=False Wx and dWx times sqrt(scale) (_cwt.py:307-311).  x 1D or
-    2D."""
+    psih(scale*xi) with the Nyquist bin halved, ifft, unpad (all L columns
+    with rpadded, _cwt.py:293-305); with l1_norm=False Wx and dWx times
+    sqrt(scale) (_cwt.py:307-311).  x 1D or 2D."""
     x = np.asarray(x, dtype=np.float64)
     N = x.shape[-1]
     xp, L, n1, _ = pad_signal(x, padtype)
@@ -151,10 +154,11 @@ def cwt(x, psih, scales, dt=1., padtype='reflect', derivative=True, l1_norm=True
         xh = xh[:, None]
     if not l1_norm:
         P = P * np.sqrt(scales)
-    Wx = np.fft.ifft(P * xh, axis=-1)[..., n1:n1 + N]
+    cols = slice(None) if rpadded else slice(n1, n1 + N)
+    Wx = np.fft.ifft(P * xh, axis=-1)[..., cols]
     if not derivative:
         return Wx, None
-    dWx = np.fft.ifft(P * xh * (1j * xi / dt), axis=-1)[..., n1:n1 + N]
+    dWx = np.fft.ifft(P * xh * (1j * xi / dt), axis=-1)[..., cols]
     return Wx, dWx
 
 

@@ -156,14 +156,14 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
     complex64 on the GPU (NumPy with astensor=False)."""
     import torch
     higher = isinstance(order, (tuple, list, range)) or order > 0
-    if rpadded:
-        raise NotImplementedError("rpadded=True is not on the GPU path")
     if not hasattr(x, 'ndim'):
         raise TypeError("`x` must be a numpy array or torch Tensor "
                         "(got %s)" % type(x))
     elif x.ndim not in (1, 2):
         raise ValueError("`x` must be 1D or 2D (got x.ndim == %s)" % x.ndim)
     grad = wants_grad(x)
+    if grad and rpadded:
+        raise NotImplementedError("gradients of cwt(rpadded=True) are not supported")
     if not grad:
         x = _nan_checks(x, nan_checks)
     if not isinstance(scales, str):
@@ -176,12 +176,25 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
     scales = process_scales(scales, N, wavelet, nv=nv)
     scales64 = np.asarray(scales, dtype=np.float64).ravel()
     na = len(scales64)
+    # rpadded=True (_cwt.py:293-305): the signal is padded on the device and
+    # the plan runs unpadded over the padded length, a circular transform
+    # whose every column is the reference's padded Wx
+    pt = None if rpadded else padtype
+
+    def prep(x_dev):
+        if not rpadded or padtype is None:
+            return x_dev
+        L, n1, pad = _padding(N, padtype)
+        xp = torch.empty(x_dev.shape[:-1] + (L,), dtype=torch.float32, device=x_dev.device)
+        _lib.pad_signal(x_dev, L, n1, pad, xp)
+        return xp
 
     def run(x_dev):
-        shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
+        n = x_dev.shape[-1]
+        shape = (na, n) if x_dev.ndim == 1 else (x_dev.shape[0], na, n)
         Wx = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
         dWx = torch.empty_like(Wx) if derivative else None
-        plan = run_cwt(x_dev, wavelet, scales64, dt, padtype, Wx, dWx=dWx,
+        plan = run_cwt(x_dev, wavelet, scales64, dt, pt, Wx, dWx=dWx,
                        l2=not l1_norm)
         return Wx, ((dWx,) if derivative else ()), plan
 
@@ -193,12 +206,13 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
 
         def run_orders(ks):
             def run_h(x_dev):
-                shape = (na, N) if x_dev.ndim == 1 else (x_dev.shape[0], na, N)
+                n = x_dev.shape[-1]
+                shape = (na, n) if x_dev.ndim == 1 else (x_dev.shape[0], na, n)
                 Ws, dWs, plans = [], [], []
                 for k in ks:
                     Wk = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
                     dWk = torch.empty_like(Wk) if derivative else None
-                    plans.append(run_cwt(x_dev, wavelet, scales64, dt, padtype, Wk, dWx=dWk,
+                    plans.append(run_cwt(x_dev, wavelet, scales64, dt, pt, Wk, dWx=dWk,
                                          kind_params=gmw_order_params(wavelet, k)))
                     Ws.append(Wk)
                     dWs.append(dWk)
@@ -211,7 +225,7 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
 
         groups = [[k] for k in orders] if (listed and not avg) else [orders]
         x_dev = (differentiable_input(x, wavelet.dtype) if grad
-                 else signal_to_device(x, wavelet.dtype))
+                 else prep(signal_to_device(x, wavelet.dtype)))
         Ws, dWs = [], []
         for ks in groups:
             if grad:
@@ -235,7 +249,7 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
         outs = CwtFunction.apply(x_dev, run)
         Wx, dWx = outs[0], (outs[1] if derivative else None)
     else:
-        x_dev = signal_to_device(x, wavelet.dtype)
+        x_dev = prep(signal_to_device(x, wavelet.dtype))
         Wx, extra, _ = run(x_dev)
         dWx = extra[0] if derivative else None
     scales_out = device_constant(scales64.astype(wavelet.dtype), torch.float32,

@@ -115,6 +115,8 @@ _SIGS = {
                                   c_double, c_void_p, c_void_p]),
     'ssqk_trigdiff': (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int,
                               c_double, c_void_p, c_void_p]),
+    'ssqk_pad_signal': (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int,
+                                c_void_p, c_void_p]),
     'ssqk_ridge_workspace_size': (c_int, [c_int64, c_int64, ctypes.POINTER(c_size_t)]),
     'ssqk_ridge_track': (c_int, [c_void_p, c_int64, c_int64, c_void_p, ctypes.c_float,
                                  ctypes.c_float, c_int, c_int, c_void_p, c_void_p, c_void_p,
@@ -395,6 +397,13 @@ def ridge_track(Tf, s, penalty, eps, n_ridges, bw, ridge, ridge_e=None):
     check(load().ssqk_ridge_track(_ptr(Tf), nf, N, _ptr(s), float(penalty), float(eps),
                                   int(n_ridges), int(bw), _ptr(ridge), _ptr(ridge_e),
                                   _ptr(ws), _stream(Tf.device)))
+
+
+def pad_signal(x, L, n1, pad, out):
+    """ssqk_pad_signal: x float32 [rows, N] -> out float32 [rows, L]."""
+    rows = int(np.prod(x.shape[:-1]))
+    check(load().ssqk_pad_signal(_ptr(x), rows, x.shape[-1], int(L), int(n1), PAD_CODES[pad],
+                                 _ptr(out), _stream(x.device)))
 
 
 def trigdiff(A, N_out, L, n1, pad, fs, out):

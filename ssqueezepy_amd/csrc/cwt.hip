@@ -1728,6 +1728,39 @@ extern "C" int ssqk_trigdiff(const void* A, int64_t rows, int64_t N_in, int64_t 
   return SSQK_OK;
 }
 
+// out[r, n] = x[r, pad_src(n)] over the padded length (zero where the mode
+// gives no source); one thread per output sample, 4 per lane.
+__global__ __launch_bounds__(256) void k_pad_signal(const float* __restrict__ x, int64_t N,
+                                                    int64_t L, int64_t n1, int type,
+                                                    float* __restrict__ out) {
+  const float* xr = x + (int64_t)blockIdx.y * N;
+  float* o = out + (int64_t)blockIdx.y * L;
+  const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t n = base + u * 256;
+    if (n < L) {
+      const int64_t s = pad_src(n, N, n1, type);
+      o[n] = s >= 0 ? xr[s] : 0.f;
+    }
+  }
+}
+
+extern "C" int ssqk_pad_signal(const float* x, int64_t rows, int64_t N, int64_t L, int64_t n1,
+                               int pad_type, float* out, void* stream) {
+  if (!x || !out || rows < 1 || rows > 65535 || N < 1 || n1 < 0 || n1 + N > L ||
+      pad_type < SSQK_PAD_REFLECT || pad_type > SSQK_PAD_WRAP) {
+    ssqk_host::set_error("ssqk_pad_signal: bad arguments (rows %lld, N %lld, L %lld, n1 %lld, "
+                         "pad %d)", (long long)rows, (long long)N, (long long)L,
+                         (long long)n1, pad_type);
+    return SSQK_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_pad_signal, dim3((unsigned)((L + 1023) / 1024), (unsigned)rows), dim3(256),
+                     0, (hipStream_t)stream, x, N, L, n1, pad_type, out);
+  SSQK_CHECK(hipGetLastError());
+  return SSQK_OK;
+}
+
 extern "C" int ssqk_cwt_plan_destroy(ssqk_cwt_plan p) {
   if (!p) return SSQK_OK;
   (void)hipFree(p->d_scales);

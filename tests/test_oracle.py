@@ -29,6 +29,7 @@ def test_cwt_matches_bruteforce_dft_convolution():
     scales = np.array([1.1, 2.3, 5.0])
     psih = O.psih_fn('gmw', gamma=3., beta=60.)
     Wx, dWx = O.cwt(x, psih, scales, dt=0.5)
+    Wr, dWr = O.cwt(x, psih, scales, dt=0.5, rpadded=True)
     xp, L, n1, _ = O.pad_signal(x, 'reflect')
     n = np.arange(L)
     xi = O.xi_grid(L)
@@ -42,6 +43,8 @@ def test_cwt_matches_bruteforce_dft_convolution():
         yd = E @ (H * X * 1j * xi / 0.5) / L
         assert np.allclose(Wx[i], y[n1:n1 + len(x)], atol=1e-12)
         assert np.allclose(dWx[i], yd[n1:n1 + len(x)], atol=1e-11)
+        # rpadded=True: every column of the circular convolution
+        assert np.allclose(Wr[i], y, atol=1e-12) and np.allclose(dWr[i], yd, atol=1e-11)
 
 
 def test_wavelet_closed_forms():
