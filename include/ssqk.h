@@ -35,6 +35,11 @@ extern "C" {
  * wavelet tables (the adjoint uses the same tables).  ssqk_psih rejects
  * it. */
 #define SSQK_WAVELET_NORM_L2 0x100
+/* OR'ed into the wavelet kind of a CWT plan: the Nyquist bin of psih not
+ * halved, the wavelet(scale=, N=) of the double-integral icwt
+ * (_cwt.py:456-476, wavelets.py:62-78 with its default nohalf=True); for
+ * ssqk_cwt_adjoint over padded rows.  ssqk_psih rejects it. */
+#define SSQK_WAVELET_FULL_NYQUIST 0x200
 
 #define SSQK_PAD_REFLECT 0
 #define SSQK_PAD_ZERO 1
@@ -231,6 +236,16 @@ int ssqk_stft_exec(const float* x, int64_t batch, int64_t N, int64_t n_fft,
  * rows <= 65535. */
 int ssqk_pad_signal(const float* x, int64_t rows, int64_t N, int64_t L, int64_t n1,
                     int pad_type, float* out, void* stream);
+
+/* Pad complex64 rows A [rows, N] to out [rows, L] as ssqk_pad_signal does,
+ * each row times row_factor[r] (float32; NULL for none): padsignal of Wx
+ * (utils/common.py:54-158) in the double-integral icwt (_cwt.py:456-476),
+ * the factor its norm(scale) and the log segment's constant.  With a plan
+ * of length L, no padding and SSQK_WAVELET_FULL_NYQUIST, ssqk_cwt_adjoint
+ * of these rows is sum_r Re ifft(psih_r fft(out_r)), the icwt before its
+ * unpadding. */
+int ssqk_pad_complex(const void* A, int64_t rows, int64_t N, int64_t L, int64_t n1,
+                     int pad_type, const float* row_factor, void* out, void* stream);
 
 /* Frequency-domain derivative of complex rows (complex64): A [rows, N_in]
  * padded to L (a power of two <= 8192) with pad_type at left pad n1, or,

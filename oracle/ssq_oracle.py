@@ -33,7 +33,8 @@ ssqueezepy package root).  How each function is pinned:
                     (Laguerre polynomials under the weight x^c e^-x)
   issq_full,
   invert_components,
-  icwt_1int         the reference's reconstruction thresholds
+  icwt_1int,
+  icwt_2int         the reference's reconstruction thresholds
                     (tests/reconstruction_test.py:65-123) on the float64
                     chain; components + remainder = full inversion
   ridge_track       known answer (tests/ridge_extraction_test.py:17-26);
@@ -375,6 +376,22 @@ def icwt_1int(Wx, segments):
 
 
 # ------------------------------------------------------------------ ridges
+
+def icwt_2int(Wx, psih, scales, row_factor, padtype='reflect'):
+    """Double-integral icwt (_cwt.py:456-476) in float64, each row times its
+    factor (k / norm(scale)): pad the rows of Wx, fft, times psih(scale xi)
+    (Nyquist bin whole: wavelet(scale=, N=) has nohalf=True) times
+    (-1)^n, ifft, ifftshift, real part, sum over rows, unpad."""
+    Wx = np.asarray(Wx, dtype=np.complex128)
+    N = Wx.shape[-1]
+    Wp, L, n1, _ = pad_signal(Wx, padtype)
+    P = psih(np.asarray(scales, dtype=np.float64).reshape(-1, 1) * xi_grid(L))
+    pn = (-1.) ** np.arange(L)
+    xa = np.fft.ifftshift(np.fft.ifft(np.fft.fft(Wp, axis=-1) * P * pn, axis=-1), axes=-1)
+    x = (xa.real * np.asarray(row_factor, dtype=np.float64)[:, None]).sum(axis=0)
+    return x[n1:n1 + N]
+
+
 def ridge_track(Tf, scales, penalty=2., n_ridges=1, bw=15, transform='cwt', cr=True):
     """extract_ridges (ridge_extraction.py:11-232), serial path, float32 as
     the reference evaluates a complex64 map: -log(energy / max + eps),

@@ -25,7 +25,7 @@ def _wavelet_kernel_params(wavelet):
     return (p['mu'], 0.)
 
 
-KIND_NORM_L2 = 0x100   # SSQK_WAVELET_NORM_L2
+KIND_NORM_L2 = _lib.WAVELET_NORM_L2
 
 
 def process_gmw_wavelet(wavelet, l1_norm):

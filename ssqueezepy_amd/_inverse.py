@@ -161,9 +161,11 @@ def icwt(Wx, wavelet='gmw', scales='log-piecewise', nv=None, one_int=True, x_len
     log segment and add them.  Wx: [na, N] or [batch, na, N]."""
     import torch
     from ._scales import process_scales, logscale_transition_idx
-    if not one_int:
-        raise NotImplementedError("icwt(one_int=False) (double integral) is not built")
     W = _device_tensor(Wx, 'Wx')
+    if not one_int and W.ndim == 3:
+        raise NotImplementedError("batched `Wx` requires `one_int=True`.")
+    if not one_int and rpadded:
+        raise NotImplementedError("icwt(one_int=False, rpadded=True) is not supported")
     na, n = W.shape[-2], W.shape[-1]
     x_len = x_len or n
     if not (isinstance(scales, np.ndarray) or torch.is_tensor(scales)) and nv is None:
@@ -174,6 +176,9 @@ def icwt(Wx, wavelet='gmw', scales='log-piecewise', nv=None, one_int=True, x_len
                                               get_params=True)
     scales = np.asarray(scales)
     assert len(scales) == na, "%s != %s" % (len(scales), na)
+    if not one_int:
+        return _icwt_2int(W, wavelet, scales, scaletype, nv, l1_norm, x_len, padtype, x_mean,
+                          astensor)
     Css = adm_ssq(wavelet)
     x = torch.empty(W.shape[:-2] + W.shape[-1:], dtype=torch.float32, device=W.device)
 
@@ -199,4 +204,54 @@ def icwt(Wx, wavelet='gmw', scales='log-piecewise', nv=None, one_int=True, x_len
         segment(idx, na, scales[idx:], True)
     else:
         segment(0, na, scales, False)
+    return x if astensor else x.cpu().numpy()
+
+
+def _icwt_2int(W, wavelet, scales, scaletype, nv, l1_norm, x_len, padtype, x_mean, astensor):
+    """Double-integral icwt (_cwt.py:456-476 and its constants,
+    _cwt.py:437-452): x = Re sum_i ifft(psih_i fft(pad(Wx_i))) f_i,
+    unpadded, f_i = (2 / Cpsi) ln(2^(1/nv)) / norm(s_i) (log) or
+    (2 / Cpsi) (pi / 4) / norm(s_i) (linear), Cpsi = adm_cwt; a
+    log-piecewise grid is two log segments, each adding x_mean.  The
+    reference's (-1)^n on psih and its ifftshift cancel for an even padded
+    length.  Device work: ssqk_pad_complex (pad, times f_i) and
+    ssqk_cwt_adjoint of a plan of the padded length with the whole Nyquist
+    bin (the reference's wavelet(scale=, N=))."""
+    import torch
+    from ._cwt import _padding, _wavelet_kernel_params
+    from ._scales import process_scales, logscale_transition_idx
+    na, n = W.shape[-2], W.shape[-1]
+    Cpsi = adm_cwt(wavelet)
+
+    def factors(s):
+        _, stype, _, nv_s = process_scales(s, x_len, wavelet, get_params=True)
+        s32 = np.asarray(s, dtype=np.float32).reshape(-1).astype(np.float64)
+        if stype == 'log':
+            k = (2 / Cpsi) * np.log(2 ** (1 / nv_s))
+            norm = np.ones_like(s32) if l1_norm else s32**.5
+        else:
+            k = (2 / Cpsi) * np.pi / 4
+            norm = s32 if l1_norm else s32**1.5
+        return k / norm
+
+    if scaletype == 'log-piecewise':
+        idx = logscale_transition_idx(scales)
+        f = np.concatenate([factors(scales[:idx]), factors(scales[idx:])])
+        n_seg = 2
+    else:
+        f = factors(scales)
+        n_seg = 1
+    L, n1, pad = _padding(n, padtype)
+    Wp = torch.empty((na, L), dtype=torch.complex64, device=W.device)
+    f_d = torch.as_tensor(f.astype(np.float32), device=W.device)
+    _lib.pad_complex(W, L, n1, pad, Wp, row_factor=f_d)
+    scales64 = np.asarray(scales, dtype=np.float64).reshape(-1)
+    plan = _lib.cwt_plan(W.device, L, L, 0, 'zero',
+                         wavelet.kind | _lib.WAVELET_FULL_NYQUIST,
+                         _wavelet_kernel_params(wavelet), scales64, 1.0)
+    xp = torch.empty(L, dtype=torch.float32, device=W.device)
+    plan.adjoint(xp, gWx=Wp)
+    x = xp[n1:n1 + x_len]
+    if x_mean:
+        x = x + float(x_mean) * n_seg
     return x if astensor else x.cpu().numpy()

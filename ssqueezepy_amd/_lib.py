@@ -117,6 +117,8 @@ _SIGS = {
                               c_double, c_void_p, c_void_p]),
     'ssqk_pad_signal': (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int,
                                 c_void_p, c_void_p]),
+    'ssqk_pad_complex': (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int,
+                                 c_void_p, c_void_p, c_void_p]),
     'ssqk_ridge_workspace_size': (c_int, [c_int64, c_int64, ctypes.POINTER(c_size_t)]),
     'ssqk_ridge_track': (c_int, [c_void_p, c_int64, c_int64, c_void_p, ctypes.c_float,
                                  ctypes.c_float, c_int, c_int, c_void_p, c_void_p, c_void_p,
@@ -404,6 +406,18 @@ def pad_signal(x, L, n1, pad, out):
     rows = int(np.prod(x.shape[:-1]))
     check(load().ssqk_pad_signal(_ptr(x), rows, x.shape[-1], int(L), int(n1), PAD_CODES[pad],
                                  _ptr(out), _stream(x.device)))
+
+
+WAVELET_NORM_L2 = 0x100          # SSQK_WAVELET_NORM_L2
+WAVELET_FULL_NYQUIST = 0x200     # SSQK_WAVELET_FULL_NYQUIST
+
+
+def pad_complex(A, L, n1, pad, out, row_factor=None):
+    """ssqk_pad_complex: A complex64 [rows, N] -> out complex64 [rows, L],
+    row r times row_factor[r] (float32 device tensor) if given."""
+    rows = int(np.prod(A.shape[:-1]))
+    check(load().ssqk_pad_complex(_ptr(A), rows, A.shape[-1], int(L), int(n1), PAD_CODES[pad],
+                                  _ptr(row_factor), _ptr(out), _stream(A.device)))
 
 
 def trigdiff(A, N_out, L, n1, pad, fs, out):

@@ -358,8 +358,9 @@ extern "C" int ssqk_psih(int wavelet_kind, const double* params, const double* s
   WavDev wav;
   {
     const int rc = make_wav_dev(wavelet_kind, params, &wav, nullptr);
-    if (rc == SSQK_OK && wav.l2) {
-      ssqk_host::set_error("ssqk_psih: SSQK_WAVELET_NORM_L2 applies to CWT plans only");
+    if (rc == SSQK_OK && (wav.l2 || wav.full_nyq)) {
+      ssqk_host::set_error("ssqk_psih: SSQK_WAVELET_NORM_L2 / _FULL_NYQUIST apply to CWT "
+                           "plans only");
       return SSQK_ERR_ARG;
     }
     if (rc == SSQK_ERR_ARG) {

@@ -21,6 +21,7 @@ struct WavDev {
   int nk;
   double kc[kMaxGmwOrder + 1];
   int l2;                   // psih(s xi) times sqrt(s) in the CWT tables
+  int full_nyq;             // the Nyquist bin not halved in the CWT tables
 };
 
 // psih(w) in float64 (_gmw.py:203-210, 267-293, wavelets.py:521-523).
@@ -45,12 +46,14 @@ __device__ __forceinline__ double psih_eval(const WavDev& p, double w) {
 
 // Device parameters of wavelet `kind` from its ABI parameter list (GMW:
 // gamma, beta; Morlet: mu; GMW of order k: gamma, beta, k + 1 and the
-// constants; SSQK_WAVELET_NORM_L2 in `kind` sets l2); the centre frequency
+// constants; SSQK_WAVELET_NORM_L2 / _FULL_NYQUIST in `kind` set l2 /
+// full_nyq); the centre frequency
 // of the order-0 wavelet.  0 or an error.
 inline int make_wav_dev(int kind, const double* prm, WavDev* w, double* center) {
   *w = WavDev{};
   w->l2 = (kind & SSQK_WAVELET_NORM_L2) != 0;
-  kind &= ~SSQK_WAVELET_NORM_L2;
+  w->full_nyq = (kind & SSQK_WAVELET_FULL_NYQUIST) != 0;
+  kind &= ~(SSQK_WAVELET_NORM_L2 | SSQK_WAVELET_FULL_NYQUIST);
   w->kind = kind;
   if (kind == SSQK_WAVELET_GMW || kind == SSQK_WAVELET_GMW_K) {
     const double g = prm[0], b = prm[1];

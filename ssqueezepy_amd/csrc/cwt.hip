@@ -202,12 +202,12 @@ __device__ __forceinline__ void emit(const CwtArgs& a, const OutRow& r, int n, f
 // v = xh[k] was loaded by the caller (loads are kept out of branches so that
 // they are issued together).  valid = false gives zeros.
 // psih(s xi_k) rounded to float32 with the Nyquist bin halved
-// (wavelets.py:86-95), times sqrt(s) for the L2 norm (_cwt.py:307-311);
-// the plan tabulates it once per scale.
+// (wavelets.py:86-95; whole for full_nyq), times sqrt(s) for the L2 norm
+// (_cwt.py:307-311); the plan tabulates it once per scale.
 __device__ __forceinline__ float psih_value(const WavDev& w, double s, int k, int L) {
   const double xi = (double)k * (kTwoPi / (double)L);
   double ps = psih_eval(w, s * xi);
-  if (k == (L >> 1)) ps *= 0.5;
+  if (k == (L >> 1) && !w.full_nyq) ps *= 0.5;
   if (w.l2) ps *= sqrt(s);
   return (float)ps;
 }
@@ -1757,6 +1757,44 @@ extern "C" int ssqk_pad_signal(const float* x, int64_t rows, int64_t N, int64_t 
   }
   hipLaunchKernelGGL(k_pad_signal, dim3((unsigned)((L + 1023) / 1024), (unsigned)rows), dim3(256),
                      0, (hipStream_t)stream, x, N, L, n1, pad_type, out);
+  SSQK_CHECK(hipGetLastError());
+  return SSQK_OK;
+}
+
+// out[r, n] = A[r, pad_src(n)] * f[r] for complex rows (f = 1 without a
+// factor array); 4 samples per lane.
+__global__ __launch_bounds__(256) void k_pad_complex(const float2* __restrict__ A, int64_t N,
+                                                     int64_t L, int64_t n1, int type,
+                                                     const float* __restrict__ f,
+                                                     float2* __restrict__ out) {
+  const float2* ar = A + (int64_t)blockIdx.y * N;
+  float2* o = out + (int64_t)blockIdx.y * L;
+  const float g = f ? f[blockIdx.y] : 1.f;
+  const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t n = base + u * 256;
+    if (n < L) {
+      const int64_t s = pad_src(n, N, n1, type);
+      const float2 v = s >= 0 ? ar[s] : make_float2(0.f, 0.f);
+      o[n] = make_float2(v.x * g, v.y * g);
+    }
+  }
+}
+
+extern "C" int ssqk_pad_complex(const void* A, int64_t rows, int64_t N, int64_t L, int64_t n1,
+                                int pad_type, const float* row_factor, void* out,
+                                void* stream) {
+  if (!A || !out || rows < 1 || rows > 65535 || N < 1 || n1 < 0 || n1 + N > L ||
+      pad_type < SSQK_PAD_REFLECT || pad_type > SSQK_PAD_WRAP) {
+    ssqk_host::set_error("ssqk_pad_complex: bad arguments (rows %lld, N %lld, L %lld, "
+                         "n1 %lld, pad %d)", (long long)rows, (long long)N, (long long)L,
+                         (long long)n1, pad_type);
+    return SSQK_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_pad_complex, dim3((unsigned)((L + 1023) / 1024), (unsigned)rows),
+                     dim3(256), 0, (hipStream_t)stream, (const float2*)A, N, L, n1, pad_type,
+                     row_factor, (float2*)out);
   SSQK_CHECK(hipGetLastError());
   return SSQK_OK;
 }

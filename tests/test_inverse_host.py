@@ -186,3 +186,26 @@ def test_l1_norm_false_wavelet_rules():
         process_gmw_wavelet(sq.Wavelet('gmw'), False)
     with pytest.raises(NotImplementedError):
         sq.cwt(np.zeros(64, np.float32), 'gmw', l1_norm=False)
+
+
+def test_oracle_icwt_2int_reconstructs():
+    """Double-integral icwt (_cwt.py:456-476): MAD/RMS of icwt(cwt(x)) <
+    0.1 with Cpsi = adm_cwt for the chirps and the fast transitions (GMW
+    beta = 8 and Morlet mu = 6, log scales); an error in the constant, the
+    wavelet or the shifts fails it."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._inverse import adm_cwt
+    from ssqueezepy_amd._scales import process_scales
+    for wav, pf in [(sq.Wavelet(('gmw', {'beta': 8., 'dtype': 'float64'})),
+                     O.psih_fn('gmw', beta=8.)),
+                    (sq.Wavelet(('morlet', {'mu': 6., 'dtype': 'float64'})),
+                     O.psih_fn('morlet', mu=6.))]:
+        for name in ('echirp', 'lchirp', 'fast_transitions'):
+            x, ts = SIGNALS[name](2048)
+            scales, _, _, nv = process_scales('log', len(x), wav, nv=32, get_params=True)
+            s64 = np.asarray(scales, dtype=np.float64).reshape(-1)
+            Wx, _ = O.cwt(x, pf, s64, dt=ts[1] - ts[0], derivative=False)
+            k = (2 / adm_cwt(wav)) * np.log(2 ** (1 / nv))
+            xrec = O.icwt_2int(Wx, pf, s64, np.full(len(s64), k))
+            assert mad_rms(x, xrec) < 0.1, (name, mad_rms(x, xrec))
+            assert mad_rms(x, 0.5 * xrec) > 0.3       # the constant matters
