@@ -113,6 +113,20 @@ def gmw_order_params(wavelet, k):
     return KIND_GMW_K, (g, b, float(k + 1)) + tuple(float(v) for v in consts)
 
 
+def gmw_average_params(wavelet, orders):
+    """(kind, parameters) of the mean of the order-k GMWs over ``orders``.
+    The transform is linear in psih, so the mean of the orders' transforms
+    (_cwt.py:579-590, _ssq_cwt.py:236-238) is one transform whose Laguerre
+    constants are the mean of theirs (lower orders padded with zeros)."""
+    orders = [int(k) for k in orders]
+    acc = np.zeros(max(orders) + 1)
+    for k in orders:
+        _, prm = gmw_order_params(wavelet, k)
+        acc[:k + 1] += np.asarray(prm[3:], dtype=np.float64)
+    acc /= len(orders)
+    return KIND_GMW_K, (prm[0], prm[1], float(len(acc))) + tuple(float(v) for v in acc)
+
+
 def gmw_orders(wavelet, order, average):
     """Orders and averaging of cwt_higher_order (_cwt.py:551-577)."""
     if wavelet.kind != KIND_GMW:
@@ -199,28 +213,22 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
         return Wx, ((dWx,) if derivative else ()), plan
 
     if higher:
-        # cwt_higher_order (_cwt.py:517-610): one transform per GMW order on
-        # the zeroth order's scales, averaged (or listed); the gradient of an
-        # average is the average of the orders' adjoints
+        # cwt_higher_order (_cwt.py:517-610): on the zeroth order's scales,
+        # one transform per listed order, or one with the orders' mean
+        # wavelet for an average; gradients by that plan's adjoint
         orders, avg, listed = gmw_orders(wavelet, order, average)
 
         def run_orders(ks):
+            # one plan: the order's wavelet, or the orders' mean wavelet
+            kp = gmw_average_params(wavelet, ks)
+
             def run_h(x_dev):
                 n = x_dev.shape[-1]
                 shape = (na, n) if x_dev.ndim == 1 else (x_dev.shape[0], na, n)
-                Ws, dWs, plans = [], [], []
-                for k in ks:
-                    Wk = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
-                    dWk = torch.empty_like(Wk) if derivative else None
-                    plans.append(run_cwt(x_dev, wavelet, scales64, dt, pt, Wk, dWx=dWk,
-                                         kind_params=gmw_order_params(wavelet, k)))
-                    Ws.append(Wk)
-                    dWs.append(dWk)
-                W = torch.stack(Ws).mean(0) if len(Ws) > 1 else Ws[0]
-                dW = None
-                if derivative:
-                    dW = torch.stack(dWs).mean(0) if len(dWs) > 1 else dWs[0]
-                return W, ((dW,) if derivative else ()), plans
+                W = torch.empty(shape, dtype=torch.complex64, device=x_dev.device)
+                dW = torch.empty_like(W) if derivative else None
+                plan = run_cwt(x_dev, wavelet, scales64, dt, pt, W, dWx=dW, kind_params=kp)
+                return W, ((dW,) if derivative else ()), [plan]
             return run_h
 
         groups = [[k] for k in orders] if (listed and not avg) else [orders]
@@ -259,7 +267,7 @@ def cwt(x, wavelet='gmw', scales='log-piecewise', fs=None, t=None, nv=32,
 
 
 def cwt_higher_order(x, wavelet='gmw', order=1, average=None, astensor=True, **kw):
-    """cwt with GMWs of the given order(s) (_cwt.py:517-610): one transform
-    per order on the zeroth order's scales, their mean with ``average``
-    (default for a tuple of orders), else a list."""
+    """cwt with GMWs of the given order(s) (_cwt.py:517-610) on the zeroth
+    order's scales: their mean with ``average`` (default for a tuple of
+    orders; one transform with the mean wavelet), else a list."""
     return cwt(x, wavelet, order=order, average=average, astensor=astensor, **kw)

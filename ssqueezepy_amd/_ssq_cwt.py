@@ -160,36 +160,21 @@ def ssq_cwt(x, wavelet='gmw', scales='log-piecewise', nv=None, fs=None, t=None,
         w = torch.empty(shape, dtype=torch.float32, device=dev) if get_w else None
         bins = torch.empty(shape, dtype=torch.int16, device=dev)
         plan = None
+        kp = None
         if higher:
             # _ssq_cwt.py:227-243: Wx averaged over the GMW orders on the
-            # zeroth order's scales, its derivative (the reference
-            # differentiates the padded average with trigdiff; the average
-            # of each order's frequency-domain derivative is the same
-            # linear map), then the phase transform and squeeze of ssq_cwt
-            from ._cwt import gmw_order_params, gmw_orders
+            # zeroth order's scales and its derivative (the reference
+            # differentiates the padded average with trigdiff, the same
+            # linear map), then the phase transform and squeeze: one
+            # transform with the orders' mean wavelet
+            from ._cwt import gmw_average_params, gmw_orders
             orders, _, _ = gmw_orders(wavelet, order, True)
-            Ws, dWs, plan = [], [], []
-            for k in orders:
-                Wk = torch.empty_like(Wx)
-                dWk = torch.empty_like(Wx)
-                plan.append(run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wk,
-                                    dWx=dWk, kind_params=gmw_order_params(wavelet, k)))
-                Ws.append(Wk)
-                dWs.append(dWk)
-            Wx = torch.stack(Ws).mean(0) if len(Ws) > 1 else Ws[0]
-            dW = torch.stack(dWs).mean(0) if len(dWs) > 1 else dWs[0]
-            if get_dWx:
-                dWx = dW
-            if get_w:
-                _lib.phase(Wx, dW, None, gamma, w)
-                _lib.bins_from_w(w, g['bp'], bins)
-            else:
-                _lib.bins_from_dwx(Wx, dW, None, g['bp'], gamma, bins)
-        else:
-            # get_w: phase_cwt then indexed_sum_onfly (w = inf where |Wx| <
-            # gamma); otherwise the fused ssqueeze_fast test |Wx| > gamma
-            plan = run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx,
-                           w=w, bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w)
+            kp = gmw_average_params(wavelet, orders)
+        # get_w: phase_cwt then indexed_sum_onfly (w = inf where |Wx| <
+        # gamma); otherwise the fused ssqueeze_fast test |Wx| > gamma
+        plan = run_cwt(x_dev, wavelet, g['scales64'], g['dt'], padtype, Wx, dWx=dWx,
+                       w=w, bins=bins, bp=g['bp'], gamma=gamma, strict=not get_w,
+                       kind_params=kp)
         Wsq, real = _apply_squeezing(Wx, squeezing)
         Tx = torch.empty_like(Wx)
         cst = _device_const(g, dev)

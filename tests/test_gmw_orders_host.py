@@ -36,3 +36,22 @@ def test_device_constants_match_the_oracle():
             kind, prm = gmw_order_params(wav, k)
             assert kind == KIND_GMW_K and prm[:3] == (3., beta, k + 1.)
             assert np.allclose(prm[3:], O.gmw_k_consts(3., beta, k), rtol=1e-13, atol=0)
+
+
+def test_average_constants_give_the_mean_wavelet():
+    """One wavelet with the orders' mean constants is the mean of the
+    orders' wavelets (the transform is linear in psih), for orders with
+    and without zero, against the oracle's per-order psih."""
+    import ssqueezepy_amd as sq
+    from ssqueezepy_amd._cwt import gmw_average_params
+    w = np.linspace(0.01, 12, 2001)
+    wav = sq.Wavelet('gmw')
+    wc = (60 / 3)**(1 / 3)
+    base = np.exp(-60 * np.log(wc) + wc**3 + 60 * np.log(w) - w**3)
+    for orders in [(0, 1), (0, 1, 2), (1, 3), (2,)]:
+        kind, prm = gmw_average_params(wav, orders)
+        assert kind == 2 and int(prm[2]) == max(orders) + 1 == len(prm) - 3
+        psih = np.polyval(np.asarray(prm[3:])[::-1], 2 * w**3) * base
+        ref = np.mean([O.gmw_k_psih(w, 3., 60., k) if k else O.gmw_psih(w, 3., 60.)
+                       for k in orders], axis=0)
+        assert np.allclose(psih, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
