@@ -71,17 +71,12 @@ class CwtFunction(torch.autograd.Function):
 
 
 def _adjoint_sum(plans, x_shape, device, **kw):
-    """x-gradient of a forward that is the mean of the plans' transforms
-    (one plan: the plain transform; several: GMW orders averaged): the mean
-    of their adjoints."""
+    """x-gradient of a forward by its plan's adjoint (every forward is one
+    plan: an average of GMW orders is one plan with the mean wavelet)."""
+    if len(plans) != 1:
+        raise RuntimeError("one CWT plan per differentiable forward (got %d)" % len(plans))
     gx = torch.empty(x_shape, dtype=torch.float32, device=device)
     plans[0].adjoint(gx, **kw)
-    if len(plans) > 1:
-        part = torch.empty_like(gx)
-        for p in plans[1:]:
-            p.adjoint(part, **kw)
-            gx += part
-        gx /= len(plans)
     return gx
 
 
